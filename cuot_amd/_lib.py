@@ -1,0 +1,68 @@
+"""ctypes loader for libcuot_amd.so (the gfx950 engine) and its C ABI.
+
+The library is built in-tree by ``python -m cuot_amd.build`` (or
+``__graft_entry__.build()``).  Loading fails loudly if it is missing: there is
+no CPU fallback behind this interface.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcuot_amd.so")
+
+P = C.c_void_p
+I64 = C.c_int64
+I32 = C.c_int32
+INT = C.c_int
+U64 = C.c_uint64
+
+
+class Block(C.Structure):
+    _fields_ = [("lo", C.c_uint64), ("hi", C.c_uint64)]
+
+
+class GgmParams(C.Structure):
+    _fields_ = [("t", I64), ("h", I32), ("mode", I32), ("n_out", I64), ("k0", Block), ("k1", Block)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/cuot_amd.h
+SIGNATURES = {
+    "cuot_init": (INT, [INT, C.POINTER(P)]),
+    "cuot_destroy": (INT, [P]),
+    "cuot_last_error": (C.c_char_p, []),
+    "cuot_device_sync": (INT, [P]),
+    "cuot_aes_ecb": (INT, [P, C.POINTER(Block), P, P, I64, P]),
+    "cuot_prg": (INT, [P, C.POINTER(Block), U64, U64, I64, P, P]),
+    "cuot_ggm_sender": (INT, [P, C.POINTER(GgmParams), P, P, P, P]),
+    "cuot_ggm_recver": (INT, [P, C.POINTER(GgmParams), P, P, P, P, P]),
+}
+
+_lib = None
+
+
+class CuotError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CuotError(f"{LIB_PATH} not found: build it with `python -m cuot_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = load().cuot_last_error().decode(errors="replace")
+        raise CuotError(f"{what} failed with status {rc}: {msg}")

@@ -1,0 +1,32 @@
+// kernels.h -- internal kernel argument structs and launchers (not ABI).
+#pragma once
+
+#include "common.h"
+
+// GGM expansion of t trees of depth h (one 256-lane workgroup per tree).
+struct GgmKArgs {
+  const uint32_t* T0;    // AES T-table in global memory
+  uint32_t rk[88];       // key-0 then key-1 AES-128 schedules (little-endian words)
+  int64_t t;
+  int h;
+  int mode;              // CUOT_MODE_SOT / CUOT_MODE_FERRET
+  int64_t n_out;         // leaves with global index >= n_out are not written
+  const blk* roots;      // sender: [t]
+  blk* leaves;           // [min(n_out, t<<h)]
+  blk* sums;             // sender out: [t][h][2] (xor of left/right children per depth)
+  const uint8_t* choice; // receiver: [t][h], 1 = chose the right (odd) side
+  const blk* csum;       // receiver: [t][h] chosen level sums
+  const blk* fin;        // receiver: [t] SOT final-row value / Ferret secret sum
+};
+hipError_t cuot_launch_ggm(const GgmKArgs& a, bool receiver, hipStream_t st);
+
+// AES-128 ECB / CTR streams.  T0: device AES table (cuot_device_T0()).
+const uint32_t* cuot_device_T0();
+hipError_t cuot_launch_aes_ecb(const uint32_t rk[44], const blk* in, blk* out, int64_t n, hipStream_t st);
+// out[i] = AES_key({lo = start + i, hi = stream})
+hipError_t cuot_launch_prg(const uint32_t rk[44], uint64_t stream, uint64_t start, int64_t n, blk* out,
+                           hipStream_t st);
+
+// Host helpers (aes_host.cpp).
+void cuot_aes_expand_host(const uint8_t key[16], uint32_t rk[44]);
+void cuot_aes_T0_host(uint32_t T0[256]);

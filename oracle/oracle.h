@@ -1,0 +1,114 @@
+/*
+ * oracle.h -- CPU restatement of the silent-OT correlation path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The
+ * product (cuot_amd) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line whose behaviour it restates
+ * (paths relative to the reference repository root).  Where the reference
+ * GPU code deviates from the protocol definition (non-FIPS key schedule,
+ * missing feed-forward, float FFT, ...) the oracle follows the protocol
+ * definition that the CPU code (emp-ot) and the workload description fix.
+ *
+ * Pinning status (what checks each function independently of its own text):
+ *   or_aes_*         FIPS-197 App. A.1 / C.1 vectors, OpenSSL EVP ECB,
+ *                    portable vs AES-NI implementations, and the reference's
+ *                    vendored cryptoTools AES (oracle/_ref) at random keys.
+ *   or_prg           defined here (seed derivation of the workload); pinned
+ *                    through or_aes_* only.
+ *   or_ggm_*         brute-force recursive GGM on tiny trees; level-sum
+ *                    invariants; sender/receiver puncture invariant.
+ *   or_sot_*         w = v xor e*Delta invariant; two independent receiver
+ *                    restatements (SOT fill rule vs emp-ot reconstruction).
+ *   or_qc_*          O(n^2) brute force vs CLMUL Karatsuba vs special cases
+ *                    (a = 1, a = x^s), linearity.
+ *   or_ferret_lpn    hand-checked small case; partition invariance for
+ *                    rows handled by __compute4 vs __compute1 forms.
+ *   or_bit_transpose involution and per-bit brute force.
+ *   emp-tool internals (makeBlock order, ParaEnc key assignment) are read
+ *   from call sites only: parity unpinned against emp-tool itself, which is
+ *   not present.
+ */
+#ifndef CUOT_ORACLE_H
+#define CUOT_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 128-bit block in memory byte order: bytes 0..7 = lo (little endian),
+ * bytes 8..15 = hi.  emp-tool makeBlock(hi, lo) == {lo, hi}
+ * (spcot_sender.h:118 clears bit 0 of byte 0 through makeBlock(~0, ~0-1)). */
+typedef struct {
+  uint64_t lo, hi;
+} orb;
+
+/* ---- AES-128 (FIPS-197) ------------------------------------------------ */
+void or_aes_expand(const orb *key, uint8_t rk[176]);
+/* impl: 0 = portable byte-oriented, 1 = AES-NI (falls back to 0 if absent) */
+void or_aes_ecb(const uint8_t rk[176], const orb *in, orb *out, size_t n, int impl);
+int or_have_aesni(void);
+
+/* ---- seed derivation (workload definition) ------------------------------ */
+/* out[i] = AES_seed( {lo = start + i, hi = stream} ) */
+void or_prg(const orb *seed, uint64_t stream, uint64_t start, size_t count, orb *out);
+
+/* ---- GGM tree, two-key PRG (twokeyprp.h:7-63) -------------------------- */
+/* One tree of depth h from root.  leaves: 2^h blocks.  lsum/rsum: h blocks,
+ * lsum[d] = xor of even nodes at depth d+1, rsum[d] = xor of odd nodes. */
+void or_ggm_sender_tree(const uint8_t rk0[176], const uint8_t rk1[176], orb root,
+                        int h, orb *leaves, orb *lsum, orb *rsum);
+
+/* Ferret MPCOT, emp-ot semantics, t trees of depth h, leaves truncated to
+ * n_out (<= t*2^h).  Sender: spcot_sender.h:115-152; receiver:
+ * spcot_recver.h:53-95.  b[t*h + d] = receiver choice bit at level d. */
+void or_ferret_mpcot_sender(const orb *roots, int64_t t, int h, orb delta, int64_t n_out,
+                            orb *leaves, orb *lsum, orb *rsum, orb *secret_sum, int nthreads);
+void or_ferret_mpcot_recver(int64_t t, int h, const uint8_t *b, const orb *csum,
+                            const orb *secret_sum, int64_t n_out, orb *leaves, int nthreads);
+
+/* SOT seed expansion (silent/sender.cu:166-198, silent/recver.cu:307-372)
+ * with the config's child = AES_k(s) xor s.  m0b/m1b: base-OT messages
+ * [h][t]; c[d*t + tr] receiver choice bits; M0/M1 out: [h+1][t]. */
+void or_sot_sender_expand(const uint8_t rk0[176], const uint8_t rk1[176], const orb *roots,
+                          int64_t t, int h, orb delta, const orb *m0b, const orb *m1b,
+                          int64_t n_leaves, orb *leaves, orb *M0, orb *M1, int nthreads);
+void or_sot_recver_expand(const uint8_t rk0[176], const uint8_t rk1[176], int64_t t, int h,
+                          const uint8_t *c, const orb *M0, const orb *M1, const orb *m0b,
+                          const orb *m1b, int64_t n_leaves, orb *leaves, int nthreads);
+
+/* ---- quasi-cyclic encoder (config semantics, not quasi_cyclic.cu's float
+ * path): y_i = x_i xor  XOR_{j : a_j = 1} x_{n + ((i - j) mod n)},  i < n.
+ * x: 2n blocks, a: n bits packed LSB-first, y: n blocks. */
+void or_qc_encode_brute(const orb *x, const uint8_t *a, int64_t n, orb *y);
+void or_qc_encode_fast(const orb *x, const uint8_t *a, int64_t n, orb *y, int nthreads);
+/* bits version used by the sparse encoder: e: 2n bits, out: n bits */
+void or_qc_encode_bits_brute(const uint8_t *e, const uint8_t *a, int64_t n, uint8_t *out);
+/* sparse encode (quasi_cyclic.cu:130-161 semantics on the config's code):
+ * noise positions pos[0..w) in [0, 2n); out: n bits */
+void or_qc_sparse(const int64_t *pos, int64_t w, const uint8_t *a, int64_t n, uint8_t *out);
+
+/* ---- binary polynomial product (test helper of or_qc_encode_fast) ------- */
+void or_bpmul(const uint64_t *a, const uint64_t *b, int64_t m, uint64_t *out);
+
+/* ---- 128-row bit transpose (gpu/gpu_ops.cu:31-70 bit order) ------------- */
+/* rows[k][m/8] bit (m%8) = bit k of block m (bit k = byte k/8, bit k%8). N % 8 == 0 */
+void or_bit_transpose(const orb *in, int64_t N, uint8_t *rows);
+void or_bit_untranspose(const uint8_t *rows, int64_t N, orb *out);
+
+/* ---- Ferret 10-local primal LPN (lpn_f2.h:155-188) --------------------- */
+/* nn[i] ^= XOR_j kk[idx(i,j)] for i in [0, n); PRP keyed with seed;
+ * ntasks equal-width tasks as in LpnF2::bench (lpn_f2.h:209-221). */
+void or_ferret_lpn(orb *nn, int64_t n, const orb *kk, int64_t k, orb seed, int d, int ntasks,
+                   int nthreads);
+/* indices only (for tests): idx[i*d + j] */
+void or_ferret_lpn_indices(int64_t n, int64_t k, orb seed, int d, int ntasks, uint32_t *idx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,230 @@
+"""ctypes bindings for the CPU oracle (oracle/oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the cuot_amd product.
+
+Blocks are numpy uint64 arrays of shape (N, 2) holding [lo, hi] -- the same
+memory layout as a 16-byte block in byte order (bytes 0..7 = lo).
+Bit vectors are numpy uint8 arrays packed LSB-first (bit i = byte i//8, bit i%8).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+P = C.c_void_p
+I64 = C.c_int64
+INT = C.c_int
+
+
+class ORB(C.Structure):
+    _fields_ = [("lo", C.c_uint64), ("hi", C.c_uint64)]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.or_aes_expand.argtypes = [P, P]
+        L.or_aes_ecb.argtypes = [P, P, P, C.c_size_t, INT]
+        L.or_have_aesni.restype = INT
+        L.or_prg.argtypes = [P, C.c_uint64, C.c_uint64, C.c_size_t, P]
+        L.or_ggm_sender_tree.argtypes = [P, P, ORB, INT, P, P, P]
+        L.or_ferret_mpcot_sender.argtypes = [P, I64, INT, ORB, I64, P, P, P, P, INT]
+        L.or_ferret_mpcot_recver.argtypes = [I64, INT, P, P, P, I64, P, INT]
+        L.or_sot_sender_expand.argtypes = [P, P, P, I64, INT, ORB, P, P, I64, P, P, P, INT]
+        L.or_sot_recver_expand.argtypes = [P, P, I64, INT, P, P, P, P, P, I64, P, INT]
+        L.or_qc_encode_brute.argtypes = [P, P, I64, P]
+        L.or_qc_encode_fast.argtypes = [P, P, I64, P, INT]
+        L.or_qc_encode_bits_brute.argtypes = [P, P, I64, P]
+        L.or_qc_sparse.argtypes = [P, I64, P, I64, P]
+        L.or_bpmul.argtypes = [P, P, I64, P]
+        L.or_bit_transpose.argtypes = [P, I64, P]
+        L.or_bit_untranspose.argtypes = [P, I64, P]
+        L.or_ferret_lpn.argtypes = [P, I64, P, I64, ORB, INT, INT, INT]
+        L.or_ferret_lpn_indices.argtypes = [I64, I64, ORB, INT, INT, P]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(P)
+
+
+def blocks(n: int) -> np.ndarray:
+    return np.zeros((n, 2), dtype=np.uint64)
+
+
+def as_orb(b) -> ORB:
+    b = np.asarray(b, dtype=np.uint64).reshape(2)
+    return ORB(int(b[0]), int(b[1]))
+
+
+def block_from_bytes(bs: bytes) -> np.ndarray:
+    return np.frombuffer(bytes(bs), dtype=np.uint64).reshape(-1, 2).copy()
+
+
+def block_to_bytes(b: np.ndarray) -> bytes:
+    return np.ascontiguousarray(b, dtype=np.uint64).tobytes()
+
+
+# ---- AES --------------------------------------------------------------------
+def aes_expand(key: np.ndarray) -> np.ndarray:
+    rk = np.zeros(176, dtype=np.uint8)
+    k = np.ascontiguousarray(key, dtype=np.uint64).reshape(1, 2)
+    lib().or_aes_expand(_p(k), _p(rk))
+    return rk
+
+
+def aes_ecb(key: np.ndarray, x: np.ndarray, impl: int = 1) -> np.ndarray:
+    rk = aes_expand(key)
+    x = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, 2)
+    out = blocks(x.shape[0])
+    lib().or_aes_ecb(_p(rk), _p(x), _p(out), x.shape[0], impl)
+    return out
+
+
+def have_aesni() -> bool:
+    return bool(lib().or_have_aesni())
+
+
+def prg(seed: np.ndarray, stream: int, start: int, count: int) -> np.ndarray:
+    out = blocks(count)
+    s = np.ascontiguousarray(seed, dtype=np.uint64).reshape(1, 2)
+    lib().or_prg(_p(s), stream, start, count, _p(out))
+    return out
+
+
+# ---- GGM ---------------------------------------------------------------------
+def ggm_sender_tree(k0, k1, root, h):
+    rk0, rk1 = aes_expand(k0), aes_expand(k1)
+    leaves = blocks(1 << h)
+    ls, rs = blocks(h), blocks(h)
+    lib().or_ggm_sender_tree(_p(rk0), _p(rk1), as_orb(root), h, _p(leaves), _p(ls), _p(rs))
+    return leaves, ls, rs
+
+
+def ferret_mpcot_sender(roots, t, h, delta, n_out, nthreads=1):
+    roots = np.ascontiguousarray(roots, dtype=np.uint64)
+    leaves = blocks(n_out)
+    ls, rs, ss = blocks(t * h), blocks(t * h), blocks(t)
+    lib().or_ferret_mpcot_sender(_p(roots), t, h, as_orb(delta), n_out, _p(leaves), _p(ls), _p(rs),
+                                 _p(ss), nthreads)
+    return leaves, ls, rs, ss
+
+
+def ferret_mpcot_recver(t, h, b, csum, secret_sum, n_out, nthreads=1):
+    b = np.ascontiguousarray(b, dtype=np.uint8)
+    csum = np.ascontiguousarray(csum, dtype=np.uint64)
+    secret_sum = np.ascontiguousarray(secret_sum, dtype=np.uint64)
+    leaves = blocks(n_out)
+    lib().or_ferret_mpcot_recver(t, h, _p(b), _p(csum), _p(secret_sum), n_out, _p(leaves), nthreads)
+    return leaves
+
+
+def sot_sender_expand(k0, k1, roots, t, h, delta, m0b, m1b, n_leaves, nthreads=1):
+    rk0, rk1 = aes_expand(k0), aes_expand(k1)
+    roots = np.ascontiguousarray(roots, dtype=np.uint64)
+    m0b = np.ascontiguousarray(m0b, dtype=np.uint64)
+    m1b = np.ascontiguousarray(m1b, dtype=np.uint64)
+    leaves = blocks(n_leaves)
+    M0, M1 = blocks((h + 1) * t), blocks((h + 1) * t)
+    lib().or_sot_sender_expand(_p(rk0), _p(rk1), _p(roots), t, h, as_orb(delta), _p(m0b), _p(m1b),
+                               n_leaves, _p(leaves), _p(M0), _p(M1), nthreads)
+    return leaves, M0, M1
+
+
+def sot_recver_expand(k0, k1, t, h, c, M0, M1, m0b, m1b, n_leaves, nthreads=1):
+    rk0, rk1 = aes_expand(k0), aes_expand(k1)
+    args = [np.ascontiguousarray(v, dtype=np.uint64) for v in (M0, M1, m0b, m1b)]
+    c = np.ascontiguousarray(c, dtype=np.uint8)
+    leaves = blocks(n_leaves)
+    lib().or_sot_recver_expand(_p(rk0), _p(rk1), t, h, _p(c), *[_p(v) for v in args], n_leaves,
+                               _p(leaves), nthreads)
+    return leaves
+
+
+# ---- QC ----------------------------------------------------------------------
+def qc_encode_brute(x, a_bits, n):
+    x = np.ascontiguousarray(x, dtype=np.uint64)
+    a_bits = np.ascontiguousarray(a_bits, dtype=np.uint8)
+    y = blocks(n)
+    lib().or_qc_encode_brute(_p(x), _p(a_bits), n, _p(y))
+    return y
+
+
+def qc_encode_fast(x, a_bits, n, nthreads=1):
+    x = np.ascontiguousarray(x, dtype=np.uint64)
+    a_bits = np.ascontiguousarray(a_bits, dtype=np.uint8)
+    y = blocks(n)
+    lib().or_qc_encode_fast(_p(x), _p(a_bits), n, _p(y), nthreads)
+    return y
+
+
+def qc_encode_bits_brute(e_bits, a_bits, n):
+    out = np.zeros((n + 7) // 8, dtype=np.uint8)
+    lib().or_qc_encode_bits_brute(_p(np.ascontiguousarray(e_bits, dtype=np.uint8)),
+                                  _p(np.ascontiguousarray(a_bits, dtype=np.uint8)), n, _p(out))
+    return out
+
+
+def qc_sparse(pos, a_bits, n):
+    pos = np.ascontiguousarray(pos, dtype=np.int64)
+    out = np.zeros((n + 7) // 8, dtype=np.uint8)
+    lib().or_qc_sparse(_p(pos), pos.shape[0], _p(np.ascontiguousarray(a_bits, dtype=np.uint8)), n, _p(out))
+    return out
+
+
+def bpmul(a_words, b_words):
+    a = np.ascontiguousarray(a_words, dtype=np.uint64)
+    b = np.ascontiguousarray(b_words, dtype=np.uint64)
+    m = a.shape[0]
+    out = np.zeros(2 * m, dtype=np.uint64)
+    lib().or_bpmul(_p(a), _p(b), m, _p(out))
+    return out
+
+
+# ---- transpose ---------------------------------------------------------------
+def bit_transpose(x):
+    x = np.ascontiguousarray(x, dtype=np.uint64)
+    N = x.shape[0]
+    rows = np.zeros((128, N // 8), dtype=np.uint8)
+    lib().or_bit_transpose(_p(x), N, _p(rows))
+    return rows
+
+
+def bit_untranspose(rows, N):
+    rows = np.ascontiguousarray(rows, dtype=np.uint8)
+    out = blocks(N)
+    lib().or_bit_untranspose(_p(rows), N, _p(out))
+    return out
+
+
+# ---- Ferret LPN --------------------------------------------------------------
+def ferret_lpn(nn, kk, seed, d=10, ntasks=1, nthreads=1):
+    nn = np.array(nn, dtype=np.uint64, copy=True, order="C")
+    kk = np.ascontiguousarray(kk, dtype=np.uint64)
+    lib().or_ferret_lpn(_p(nn), nn.shape[0], _p(kk), kk.shape[0], as_orb(seed), d, ntasks, nthreads)
+    return nn
+
+
+def ferret_lpn_indices(n, k, seed, d=10, ntasks=1):
+    idx = np.zeros(n * d, dtype=np.uint32)
+    lib().or_ferret_lpn_indices(n, k, as_orb(seed), d, ntasks, _p(idx))
+    return idx.reshape(n, d)

@@ -13,12 +13,12 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import Block, CuotError, GgmParams, check, load
+from ._lib import Block, CuotError, GgmParams, LpnParams, check, load
 
 MODE_SOT = 0
 MODE_FERRET = 1
 
-__all__ = ["Context", "Block", "GgmParams", "CuotError", "MODE_SOT", "MODE_FERRET", "to_block", "load"]
+__all__ = ["Context", "Block", "GgmParams", "LpnParams", "CuotError", "MODE_SOT", "MODE_FERRET", "to_block", "load"]
 
 
 def to_block(b) -> Block:
@@ -82,3 +82,40 @@ class Context:
     def ggm_recver(self, params: GgmParams, choice, csum, fin, leaves, stream=None):
         check(self.lib.cuot_ggm_recver(self.handle, C.byref(params), _ptr(choice), _ptr(csum), _ptr(fin),
                                        _ptr(leaves), _stream_ptr(stream)), "cuot_ggm_recver")
+
+    def lpn(self, params: LpnParams, noise, kk, out, stream=None):
+        check(self.lib.cuot_lpn(self.handle, C.byref(params), _ptr(noise), _ptr(kk), _ptr(out),
+                                _stream_ptr(stream)), "cuot_lpn")
+
+    def lpn_dual(self, params: LpnParams, noise_s, kk_s, out_s, noise_r, kk_r, out_r, stream=None):
+        check(self.lib.cuot_lpn_dual(self.handle, C.byref(params), _ptr(noise_s), _ptr(kk_s), _ptr(out_s),
+                                     _ptr(noise_r), _ptr(kk_r), _ptr(out_r), _stream_ptr(stream)),
+              "cuot_lpn_dual")
+
+    # ---- quasi-cyclic code -----------------------------------------------------
+    def qc_ahat_len(self, n: int) -> int:
+        return int(self.lib.cuot_qc_ahat_len(n))
+
+    def qc_scratch_bytes(self, n: int) -> int:
+        return int(self.lib.cuot_qc_scratch_bytes(n))
+
+    def qc_prepare(self, n, a_bits, ahat, scratch, stream=None):
+        check(self.lib.cuot_qc_prepare(self.handle, n, _ptr(a_bits), _ptr(ahat), _ptr(scratch),
+                                       _stream_ptr(stream)), "cuot_qc_prepare")
+
+    def qc_encode(self, n, ahat, x, y, scratch, stream=None):
+        check(self.lib.cuot_qc_encode(self.handle, n, _ptr(ahat), _ptr(x), _ptr(y), _ptr(scratch),
+                                      _stream_ptr(stream)), "cuot_qc_encode")
+
+    def qc_sparse(self, n, a_bits, pos, out_bits, stream=None):
+        w = pos.numel()
+        check(self.lib.cuot_qc_sparse(self.handle, n, _ptr(a_bits), _ptr(pos) if w else None, w, _ptr(out_bits),
+                                      _stream_ptr(stream)), "cuot_qc_sparse")
+
+    def bit_transpose(self, x, rows, stream=None):
+        check(self.lib.cuot_bit_transpose(self.handle, _ptr(x), x.shape[0], _ptr(rows), _stream_ptr(stream)),
+              "cuot_bit_transpose")
+
+    def bit_untranspose(self, rows, n_blocks, out, stream=None):
+        check(self.lib.cuot_bit_untranspose(self.handle, _ptr(rows), n_blocks, _ptr(out), _stream_ptr(stream)),
+              "cuot_bit_untranspose")

@@ -27,6 +27,10 @@ class GgmParams(C.Structure):
     _fields_ = [("t", I64), ("h", I32), ("mode", I32), ("n_out", I64), ("k0", Block), ("k1", Block)]
 
 
+class LpnParams(C.Structure):
+    _fields_ = [("n", I64), ("k", I64), ("d", I32), ("ntasks", I32), ("seed", Block)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/cuot_amd.h
 SIGNATURES = {
     "cuot_init": (INT, [INT, C.POINTER(P)]),
@@ -37,6 +41,15 @@ SIGNATURES = {
     "cuot_prg": (INT, [P, C.POINTER(Block), U64, U64, I64, P, P]),
     "cuot_ggm_sender": (INT, [P, C.POINTER(GgmParams), P, P, P, P]),
     "cuot_ggm_recver": (INT, [P, C.POINTER(GgmParams), P, P, P, P, P]),
+    "cuot_lpn": (INT, [P, C.POINTER(LpnParams), P, P, P, P]),
+    "cuot_lpn_dual": (INT, [P, C.POINTER(LpnParams), P, P, P, P, P, P, P]),
+    "cuot_qc_ahat_len": (I64, [I64]),
+    "cuot_qc_scratch_bytes": (C.c_size_t, [I64]),
+    "cuot_qc_prepare": (INT, [P, I64, P, P, P, P]),
+    "cuot_qc_encode": (INT, [P, I64, P, P, P, P, P]),
+    "cuot_qc_sparse": (INT, [P, I64, P, P, I64, P, P]),
+    "cuot_bit_transpose": (INT, [P, P, I64, P, P]),
+    "cuot_bit_untranspose": (INT, [P, P, I64, P, P]),
 }
 
 _lib = None

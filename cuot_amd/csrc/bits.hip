@@ -1,0 +1,160 @@
+// bits.hip -- bit-level kernels around the QC code:
+//   * sparse encode of the receiver's regular noise (choice bits):
+//       b_i = e_i ^ XOR_{p in pos, p >= n} a_{(i - (p - n)) mod n},  i < n,
+//     i.e. the same code applied to the weight-t vector e (the reference's
+//     cyclic_mat_vec_prod + Mat::modp, silent/quasi_cyclic.cu:130-161, on the
+//     workload's 2n -> n code);
+//   * the 128-row bit transpose with the reference's bit order
+//     (gpu/gpu_ops.cu:31-70): row k bit m = bit k of block m, bit k = byte k/8
+//     bit k%8, LSB-first within bytes -- done as 32x32 in-register transposes
+//     across 32 lanes, staged through LDS so both sides are coalesced.
+#include "kernels.h"
+
+namespace {
+
+// 32 bits of the cyclic n-bit vector a starting at bit `start` (bit b of the
+// result = a[(start + b) mod n]).  a is padded with at least one zero word.
+CUOT_DEV uint32_t cyc_window(const uint32_t* __restrict__ a, int64_t n, int64_t start) {
+  uint32_t out = 0;
+  int got = 0;
+  while (got < 32) {
+    const int64_t s = start % n;
+    int64_t avail = n - s;  // bits before wrap
+    int take = (int)((32 - got) < avail ? (32 - got) : avail);
+    const int64_t wi = s >> 5;
+    const int sh = (int)(s & 31);
+    uint64_t two = (uint64_t)a[wi] | ((uint64_t)a[wi + 1] << 32);
+    uint32_t bits = (uint32_t)(two >> sh);
+    if (take < 32) bits &= (1u << take) - 1u;
+    out |= bits << got;
+    got += take;
+    start = s + take;
+  }
+  return out;
+}
+
+__global__ void sparse_encode_kernel(const uint32_t* __restrict__ a, int64_t n, const int64_t* __restrict__ pos,
+                                     int w, uint32_t* __restrict__ out) {
+  const int64_t nw = (n + 31) / 32;
+  for (int64_t W = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; W < nw; W += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t acc = 0;
+    const int64_t lo = W * 32;
+    for (int q = 0; q < w; q++) {
+      const int64_t p = pos[q];
+      if (p < 0 || p >= 2 * n) continue;
+      if (p < n) {
+        if ((p >> 5) == W) acc ^= 1u << (p & 31);
+      } else {
+        const int64_t s = p - n;  // out_i ^= a_{(i - s) mod n}
+        int64_t start = (lo - s) % n;
+        if (start < 0) start += n;
+        acc ^= cyc_window(a, n, start);
+      }
+    }
+    if (lo + 32 > n) acc &= (1u << (n - lo)) - 1u;
+    out[W] = acc;
+  }
+}
+
+// ---- transpose ---------------------------------------------------------------
+CUOT_DEV uint32_t transpose32(uint32_t x, int lane) {
+  const uint32_t masks[5] = {0x0000FFFFu, 0x00FF00FFu, 0x0F0F0F0Fu, 0x33333333u, 0x55555555u};
+#pragma unroll
+  for (int s = 0; s < 5; s++) {
+    const int j = 16 >> s;
+    const uint32_t m = masks[s];
+    const uint32_t xp = (uint32_t)__shfl_xor((int)x, j, 64);
+    if ((lane & j) == 0)
+      x = (x & m) | ((xp & m) << j);
+    else
+      x = ((xp >> j) & m) | (x & ~m);
+  }
+  return x;
+}
+
+// blocks -> rows.  WG: 256 lanes, 1024 blocks (32 tiles of 32 blocks).
+// rows: 128 rows of N bits, row stride N/32 words.
+__global__ __launch_bounds__(256) void transpose_fwd(const uint4* __restrict__ in, int64_t N,
+                                                     uint32_t* __restrict__ rows) {
+  __shared__ uint32_t sm[128][33];
+  const int lane = threadIdx.x & 31;
+  const int grp = threadIdx.x >> 5;  // 8 groups of 32 lanes
+  const int64_t ntile = N / 32;
+  for (int64_t t0 = (int64_t)blockIdx.x * 32; t0 < ntile; t0 += (int64_t)gridDim.x * 32) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int tl = grp * 4 + r;  // tile within the group of 32 tiles
+      const int64_t tile = t0 + tl;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (tile < ntile) v = in[tile * 32 + lane];
+      sm[0 * 32 + lane][tl] = transpose32(v.x, lane);
+      sm[1 * 32 + lane][tl] = transpose32(v.y, lane);
+      sm[2 * 32 + lane][tl] = transpose32(v.z, lane);
+      sm[3 * 32 + lane][tl] = transpose32(v.w, lane);
+    }
+    __syncthreads();
+    const int64_t rw = N / 32;  // words per row
+    for (int f = threadIdx.x; f < 128 * 32; f += 256) {
+      const int k = f >> 5, tl = f & 31;
+      if (t0 + tl < ntile) rows[(int64_t)k * rw + t0 + tl] = sm[k][tl];
+    }
+    __syncthreads();
+  }
+}
+
+// rows -> blocks
+__global__ __launch_bounds__(256) void transpose_inv(const uint32_t* __restrict__ rows, int64_t N,
+                                                     uint4* __restrict__ out) {
+  __shared__ uint32_t sm[128][33];
+  const int lane = threadIdx.x & 31;
+  const int grp = threadIdx.x >> 5;
+  const int64_t ntile = N / 32;
+  const int64_t rw = N / 32;
+  for (int64_t t0 = (int64_t)blockIdx.x * 32; t0 < ntile; t0 += (int64_t)gridDim.x * 32) {
+    for (int f = threadIdx.x; f < 128 * 32; f += 256) {
+      const int k = f >> 5, tl = f & 31;
+      sm[k][tl] = (t0 + tl < ntile) ? rows[(int64_t)k * rw + t0 + tl] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int tl = grp * 4 + r;
+      const int64_t tile = t0 + tl;
+      uint4 v;
+      v.x = transpose32(sm[0 * 32 + lane][tl], lane);
+      v.y = transpose32(sm[1 * 32 + lane][tl], lane);
+      v.z = transpose32(sm[2 * 32 + lane][tl], lane);
+      v.w = transpose32(sm[3 * 32 + lane][tl], lane);
+      if (tile < ntile) out[tile * 32 + lane] = v;
+    }
+    __syncthreads();
+  }
+}
+
+unsigned grid_for(int64_t total, int64_t cap = 1 << 16) {
+  int64_t g = (total + 255) / 256;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+}  // namespace
+
+hipError_t cuot_launch_sparse(const uint32_t* a_words, int64_t n, const int64_t* pos, int w, uint32_t* out,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(sparse_encode_kernel, dim3(grid_for((n + 31) / 32)), dim3(256), 0, st, a_words, n, pos, w,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t cuot_launch_transpose(const void* in, int64_t N, void* out, bool inverse, hipStream_t st) {
+  int64_t ntile = N / 32;
+  int64_t g = (ntile + 31) / 32;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  if (!inverse)
+    hipLaunchKernelGGL(transpose_fwd, dim3((unsigned)g), dim3(256), 0, st, (const uint4*)in, N, (uint32_t*)out);
+  else
+    hipLaunchKernelGGL(transpose_inv, dim3((unsigned)g), dim3(256), 0, st, (const uint32_t*)in, N, (uint4*)out);
+  return hipGetLastError();
+}

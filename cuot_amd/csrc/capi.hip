@@ -128,3 +128,124 @@ extern "C" int cuot_ggm_recver(cuot_ctx* ctx, const cuot_ggm_params* p, const ui
   CUOT_HIP_TRY(cuot_launch_ggm(a, true, ctx->st(stream)));
   return CUOT_OK;
 }
+
+static int lpn_prepare(cuot_ctx* ctx, const cuot_lpn_params* p, LpnLaunch& l) {
+  CUOT_CHECK_ARG(ctx && p, "ctx/params NULL");
+  CUOT_CHECK_ARG(p->d == 10, "only d = 10 is supported (LpnF2<IO, 10>)");
+  CUOT_CHECK_ARG(p->n >= 0, "n < 0");
+  CUOT_CHECK_ARG(p->k >= 1 && p->k < (1ll << 31), "k out of range");
+  CUOT_CHECK_ARG(p->ntasks >= 1, "ntasks < 1");
+  memset(&l, 0, sizeof(l));
+  cuot_aes_expand_host((const uint8_t*)&p->seed, l.rk);
+  l.n = p->n;
+  l.k = p->k;
+  l.ntasks = p->ntasks;
+  return CUOT_OK;
+}
+
+extern "C" int cuot_lpn(cuot_ctx* ctx, const cuot_lpn_params* p, const cuot_block* noise_d, const cuot_block* kk_d,
+                        cuot_block* out_d, void* stream) {
+  LpnLaunch l;
+  int rc = lpn_prepare(ctx, p, l);
+  if (rc) return rc;
+  if (p->n == 0) return CUOT_OK;
+  CUOT_CHECK_ARG(noise_d && kk_d && out_d, "buffer is NULL");
+  CUOT_HIP_TRY(hipSetDevice(ctx->device));
+  l.nroles = 1;
+  l.noise[0] = (const blk*)noise_d;
+  l.kk[0] = (const blk*)kk_d;
+  l.out[0] = (blk*)out_d;
+  CUOT_HIP_TRY(cuot_launch_lpn(l, ctx->st(stream)));
+  return CUOT_OK;
+}
+
+extern "C" int cuot_lpn_dual(cuot_ctx* ctx, const cuot_lpn_params* p, const cuot_block* noise_s_d,
+                             const cuot_block* kk_s_d, cuot_block* out_s_d, const cuot_block* noise_r_d,
+                             const cuot_block* kk_r_d, cuot_block* out_r_d, void* stream) {
+  LpnLaunch l;
+  int rc = lpn_prepare(ctx, p, l);
+  if (rc) return rc;
+  if (p->n == 0) return CUOT_OK;
+  CUOT_CHECK_ARG(noise_s_d && kk_s_d && out_s_d && noise_r_d && kk_r_d && out_r_d, "buffer is NULL");
+  CUOT_HIP_TRY(hipSetDevice(ctx->device));
+  l.nroles = 2;
+  l.noise[0] = (const blk*)noise_s_d;
+  l.kk[0] = (const blk*)kk_s_d;
+  l.out[0] = (blk*)out_s_d;
+  l.noise[1] = (const blk*)noise_r_d;
+  l.kk[1] = (const blk*)kk_r_d;
+  l.out[1] = (blk*)out_r_d;
+  CUOT_HIP_TRY(cuot_launch_lpn(l, ctx->st(stream)));
+  return CUOT_OK;
+}
+
+extern "C" int64_t cuot_qc_ahat_len(int64_t n) {
+  if (n < 1) return 0;
+  return (int64_t)1 << cuot_qc_log_len(n);
+}
+
+extern "C" size_t cuot_qc_scratch_bytes(int64_t n) {
+  if (n < 1) return 0;
+  return cuot_qc_scratch_bytes_for(n);
+}
+
+static int qc_check(int64_t n) {
+  CUOT_CHECK_ARG(n >= 1 && n <= (1ll << 34), "n out of range");
+  CUOT_CHECK_ARG(cuot_qc_log_len(n) <= 30, "n too large for the GF(2^32) transform");
+  return CUOT_OK;
+}
+
+extern "C" int cuot_qc_prepare(cuot_ctx* ctx, int64_t n, const uint8_t* a_bits_d, uint32_t* ahat_d, void* scratch_d,
+                               void* stream) {
+  CUOT_CHECK_ARG(ctx, "ctx is NULL");
+  int rc = qc_check(n);
+  if (rc) return rc;
+  CUOT_CHECK_ARG(a_bits_d && ahat_d && scratch_d, "buffer is NULL");
+  CUOT_HIP_TRY(hipSetDevice(ctx->device));
+  CUOT_HIP_TRY(cuot_launch_qc_prepare(a_bits_d, n, ahat_d, scratch_d, ctx->st(stream)));
+  return CUOT_OK;
+}
+
+extern "C" int cuot_qc_encode(cuot_ctx* ctx, int64_t n, const uint32_t* ahat_d, const cuot_block* x_d,
+                              cuot_block* y_d, void* scratch_d, void* stream) {
+  CUOT_CHECK_ARG(ctx, "ctx is NULL");
+  int rc = qc_check(n);
+  if (rc) return rc;
+  CUOT_CHECK_ARG(ahat_d && x_d && y_d && scratch_d, "buffer is NULL");
+  CUOT_HIP_TRY(hipSetDevice(ctx->device));
+  CUOT_HIP_TRY(cuot_launch_qc_encode(ahat_d, n, x_d, y_d, scratch_d, ctx->st(stream)));
+  return CUOT_OK;
+}
+
+extern "C" int cuot_qc_sparse(cuot_ctx* ctx, int64_t n, const uint8_t* a_bits_d, const int64_t* pos_d, int64_t w,
+                              uint8_t* out_bits_d, void* stream) {
+  CUOT_CHECK_ARG(ctx, "ctx is NULL");
+  CUOT_CHECK_ARG(n >= 1, "n < 1");
+  CUOT_CHECK_ARG(w >= 0 && w < (1ll << 31), "w out of range");
+  CUOT_CHECK_ARG(a_bits_d && out_bits_d && (pos_d || w == 0), "buffer is NULL");
+  CUOT_HIP_TRY(hipSetDevice(ctx->device));
+  CUOT_HIP_TRY(cuot_launch_sparse((const uint32_t*)a_bits_d, n, pos_d, (int)w, (uint32_t*)out_bits_d,
+                                  ctx->st(stream)));
+  return CUOT_OK;
+}
+
+extern "C" int cuot_bit_transpose(cuot_ctx* ctx, const cuot_block* in_d, int64_t N, uint8_t* rows_d, void* stream) {
+  CUOT_CHECK_ARG(ctx, "ctx is NULL");
+  CUOT_CHECK_ARG(N >= 0 && N % 32 == 0, "N must be a non-negative multiple of 32");
+  if (N == 0) return CUOT_OK;
+  CUOT_CHECK_ARG(in_d && rows_d, "buffer is NULL");
+  CUOT_HIP_TRY(hipSetDevice(ctx->device));
+  CUOT_HIP_TRY(cuot_launch_transpose(in_d, N, rows_d, false, ctx->st(stream)));
+  return CUOT_OK;
+}
+
+extern "C" int cuot_bit_untranspose(cuot_ctx* ctx, const uint8_t* rows_d, int64_t N, cuot_block* out_d,
+                                    void* stream) {
+  CUOT_CHECK_ARG(ctx, "ctx is NULL");
+  CUOT_CHECK_ARG(N >= 0 && N % 32 == 0, "N must be a non-negative multiple of 32");
+  if (N == 0) return CUOT_OK;
+  CUOT_CHECK_ARG(rows_d && out_d, "buffer is NULL");
+  CUOT_HIP_TRY(hipSetDevice(ctx->device));
+  CUOT_HIP_TRY(cuot_launch_transpose(rows_d, N, out_d, true, ctx->st(stream)));
+  return CUOT_OK;
+}

@@ -30,3 +30,27 @@ hipError_t cuot_launch_prg(const uint32_t rk[44], uint64_t stream, uint64_t star
 // Host helpers (aes_host.cpp).
 void cuot_aes_expand_host(const uint8_t key[16], uint32_t rk[44]);
 void cuot_aes_T0_host(uint32_t T0[256]);
+
+// Ferret 10-local LPN (lpn.hip).  nroles = 1 or 2 (both roles share indices).
+struct LpnLaunch {
+  uint32_t rk[44];  // AES-128 schedule of the LPN seed
+  int64_t n, k;
+  int ntasks;
+  int nroles;
+  const blk* noise[2];
+  const blk* kk[2];
+  blk* out[2];
+};
+hipError_t cuot_launch_lpn(const LpnLaunch& l, hipStream_t st);
+
+// Quasi-cyclic encoder (qc.hip).
+int cuot_qc_log_len(int64_t n);
+size_t cuot_qc_scratch_bytes_for(int64_t n);
+hipError_t cuot_launch_qc_prepare(const uint8_t* a_bits, int64_t n, uint32_t* ahat, void* scratch, hipStream_t st);
+hipError_t cuot_launch_qc_encode(const uint32_t* ahat, int64_t n, const void* x, void* y, void* scratch,
+                                 hipStream_t st);
+
+// Sparse encode and bit transpose (bits.hip).
+hipError_t cuot_launch_sparse(const uint32_t* a_words, int64_t n, const int64_t* pos, int w, uint32_t* out,
+                              hipStream_t st);
+hipError_t cuot_launch_transpose(const void* in, int64_t N, void* out, bool inverse, hipStream_t st);

@@ -1,0 +1,156 @@
+// lpn.hip -- Ferret's 10-local primal LPN as one fused kernel:
+//   out[i] = noise[i] ^ XOR_{j<10} kk[idx(i, j)]
+// with the indices generated on the fly by AES under the LPN seed exactly as
+// emp-ot's LpnF2 does (ferret/emp-ot/emp-ot/ferret/lpn_f2.h:155-188):
+//   rows i..i+3 (a "__compute4" group): AES(makeBlock(i, m)) for m < 10 gives
+//     40 words; row i+q uses words 10q..10q+9, idx = w & mask, minus k if >= k;
+//   the last 1..4 rows of every task ("__compute1"): AES(makeBlock(i, m)) for
+//     m < 3, idx = w[j] % k.
+// Tasks split [0, n) in equal widths n/ntasks with the last task taking the
+// remainder (lpn_f2.h:209-221).  The reference GPU path materialises the index
+// array and uses other counters (gpu/gpu_ops.cu:102-109, dev_layer.cu:146-186);
+// this kernel keeps the emp-ot definition and never writes indices to HBM.
+//
+// Both roles share the (public) indices, so one launch can produce the
+// sender's and the receiver's outputs from the same AES work.
+#include <string.h>
+
+#include "aes_dev.h"
+#include "kernels.h"
+
+namespace {
+
+constexpr int D = 10;
+
+struct LpnKArgs {
+  const uint32_t* T0;
+  uint32_t rk[44];
+  int64_t n, k;
+  uint32_t mask;
+  int64_t width;   // n / ntasks
+  int ntasks;
+  int64_t units_per_task;  // units of a full-width task
+  int64_t units_total;
+  const blk* noise[2];
+  const blk* kk[2];
+  blk* out[2];
+  int nroles;
+};
+
+// number of __compute4 groups and __compute1 rows of a task of `rows` rows
+CUOT_DEV void task_shape(int64_t rows, int64_t& g4, int64_t& t1) {
+  g4 = rows > 4 ? (rows - 4 + 3) / 4 : 0;
+  t1 = rows - 4 * g4;
+}
+
+template <int NR>
+__global__ __launch_bounds__(256) void lpn_kernel(LpnKArgs a) {
+  __shared__ uint32_t tab[CUOT_AES_LDS_WORDS];
+  aes_stage_table(tab, a.T0, threadIdx.x, 256);
+  __syncthreads();
+  const AesLds aes = aes_lds(tab);
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x; u < a.units_total; u += stride) {
+    // locate the task and the unit inside it
+    int64_t task = u / a.units_per_task;
+    if (task >= a.ntasks) task = a.ntasks - 1;
+    const int64_t start = task * a.width;
+    const int64_t end = (task == a.ntasks - 1) ? a.n : start + a.width;
+    const int64_t lu = u - task * a.units_per_task;
+    int64_t g4, t1;
+    task_shape(end - start, g4, t1);
+    if (lu < g4) {
+      const int64_t i = start + 4 * lu;
+      uint32_t w[4 * D];
+#pragma unroll
+      for (int m = 0; m < D; m++) {
+        blk c = make_uint4((uint32_t)m, 0u, (uint32_t)i, (uint32_t)((uint64_t)i >> 32));
+        blk e = aes_enc(aes, a.rk, c);
+        w[4 * m + 0] = e.x;
+        w[4 * m + 1] = e.y;
+        w[4 * m + 2] = e.z;
+        w[4 * m + 3] = e.w;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        blk acc[NR];
+#pragma unroll
+        for (int rr = 0; rr < NR; rr++) acc[rr] = a.noise[rr][i + q];
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+          uint32_t idx = w[q * D + j] & a.mask;
+          if (idx >= (uint32_t)a.k) idx -= (uint32_t)a.k;
+#pragma unroll
+          for (int rr = 0; rr < NR; rr++) acc[rr] = bxor(acc[rr], a.kk[rr][idx]);
+        }
+#pragma unroll
+        for (int rr = 0; rr < NR; rr++) a.out[rr][i + q] = acc[rr];
+      }
+    } else if (lu - g4 < t1) {
+      const int64_t i = start + 4 * g4 + (lu - g4);
+      uint32_t w[12];
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        blk c = make_uint4((uint32_t)m, 0u, (uint32_t)i, (uint32_t)((uint64_t)i >> 32));
+        blk e = aes_enc(aes, a.rk, c);
+        w[4 * m + 0] = e.x;
+        w[4 * m + 1] = e.y;
+        w[4 * m + 2] = e.z;
+        w[4 * m + 3] = e.w;
+      }
+      blk acc[NR];
+#pragma unroll
+      for (int rr = 0; rr < NR; rr++) acc[rr] = a.noise[rr][i];
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        const uint32_t idx = w[j] % (uint32_t)a.k;
+#pragma unroll
+        for (int rr = 0; rr < NR; rr++) acc[rr] = bxor(acc[rr], a.kk[rr][idx]);
+      }
+#pragma unroll
+      for (int rr = 0; rr < NR; rr++) a.out[rr][i] = acc[rr];
+    }
+  }
+}
+
+int64_t units_of(int64_t rows) {
+  int64_t g4 = rows > 4 ? (rows - 4 + 3) / 4 : 0;
+  return g4 + (rows - 4 * g4);
+}
+
+}  // namespace
+
+hipError_t cuot_launch_lpn(const LpnLaunch& l, hipStream_t st) {
+  if (l.n <= 0) return hipSuccess;
+  LpnKArgs a;
+  memset(&a, 0, sizeof(a));
+  a.T0 = cuot_device_T0();
+  memcpy(a.rk, l.rk, sizeof(a.rk));
+  a.n = l.n;
+  a.k = l.k;
+  uint32_t mask = 1;  // lpn_f2.h:136-140
+  while ((int64_t)mask < l.k) mask = (mask << 1) | 1u;
+  a.mask = mask;
+  a.ntasks = l.ntasks < 1 ? 1 : l.ntasks;
+  a.width = l.n / a.ntasks;
+  const int64_t last_rows = l.n - (int64_t)(a.ntasks - 1) * a.width;
+  a.units_per_task = units_of(a.width);
+  if (a.units_per_task < 1) a.units_per_task = 1;
+  // full-width tasks then the last one; the kernel maps u -> task by division,
+  // so the last task's units start at (ntasks-1)*units_per_task
+  a.units_total = (int64_t)(a.ntasks - 1) * a.units_per_task + units_of(last_rows);
+  a.nroles = l.nroles;
+  for (int r = 0; r < l.nroles; r++) {
+    a.noise[r] = l.noise[r];
+    a.kk[r] = l.kk[r];
+    a.out[r] = l.out[r];
+  }
+  int64_t g = (a.units_total + 255) / 256;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  if (l.nroles == 2)
+    hipLaunchKernelGGL(lpn_kernel<2>, dim3((unsigned)g), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(lpn_kernel<1>, dim3((unsigned)g), dim3(256), 0, st, a);
+  return hipGetLastError();
+}

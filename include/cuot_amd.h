@@ -85,6 +85,57 @@ int cuot_ggm_sender(cuot_ctx *ctx, const cuot_ggm_params *p, const cuot_block *r
 int cuot_ggm_recver(cuot_ctx *ctx, const cuot_ggm_params *p, const uint8_t *choice_d, const cuot_block *csum_d,
                     const cuot_block *fin_d, cuot_block *leaves_d, void *stream);
 
+/* ---- Ferret 10-local primal LPN -------------------------------------------
+ * out[i] = noise[i] ^ XOR_{j<10} kk[idx(i,j)], indices from AES under `seed`
+ * with emp-ot's counter and reduction rules (lpn_f2.h:155-188) over `ntasks`
+ * equal tasks (lpn_f2.h:209-221; emp-ot's single-thread run is ntasks = 1).
+ * out may alias noise.  Replaces cuda_primal_lpn's device work
+ * (dev_layer.cu:146-186) without materialising the index array. */
+typedef struct {
+  int64_t n;
+  int64_t k;
+  int32_t d;      /* must be 10 (LpnF2<IO, 10>, ferret_cot.h) */
+  int32_t ntasks; /* >= 1 */
+  cuot_block seed;
+} cuot_lpn_params;
+
+int cuot_lpn(cuot_ctx *ctx, const cuot_lpn_params *p, const cuot_block *noise_d, const cuot_block *kk_d,
+             cuot_block *out_d, void *stream);
+/* Both roles in one pass (sender: noise_s/kk_s, receiver: noise_r/kk_r). */
+int cuot_lpn_dual(cuot_ctx *ctx, const cuot_lpn_params *p, const cuot_block *noise_s_d,
+                  const cuot_block *kk_s_d, cuot_block *out_s_d, const cuot_block *noise_r_d,
+                  const cuot_block *kk_r_d, cuot_block *out_r_d, void *stream);
+
+/* ---- quasi-cyclic encoder (exact GF(2)) ------------------------------------
+ * y_i = x_i ^ XOR_{j: a_j = 1} x_{n + ((i - j) mod n)}, i < n, for all 128
+ * bit-rows of the blocks at once: the workload's 2n -> n circulant code.
+ * Replaces QuasiCyclic::encode_dense's float32 cuFFT path
+ * (silent/quasi_cyclic.cu:118-128) and its per-8-row batching by an exact
+ * additive FFT over GF(2^32).  a_bits_d: n bits.  ahat_d: cuot_qc_ahat_len(n)
+ * words computed once per code by cuot_qc_prepare (QuasiCyclic::QuasiCyclic,
+ * silent/quasi_cyclic.cu:51-106).  scratch_d: cuot_qc_scratch_bytes(n) bytes.
+ * y_d may alias x_d. */
+int64_t cuot_qc_ahat_len(int64_t n);
+size_t cuot_qc_scratch_bytes(int64_t n);
+int cuot_qc_prepare(cuot_ctx *ctx, int64_t n, const uint8_t *a_bits_d, uint32_t *ahat_d, void *scratch_d,
+                    void *stream);
+int cuot_qc_encode(cuot_ctx *ctx, int64_t n, const uint32_t *ahat_d, const cuot_block *x_d, cuot_block *y_d,
+                   void *scratch_d, void *stream);
+/* Receiver's choice bits: the same code on the weight-w noise vector e with
+ * ones at pos_d[0..w) (positions in [0, 2n); others ignored).  Replaces
+ * QuasiCyclic::encode_sparse (silent/quasi_cyclic.cu:130-161).  a_bits_d must
+ * be readable for ((n + 31) / 32 + 1) 32-bit words (pad with zeros).
+ * out_bits_d: (n + 31) / 32 words. */
+int cuot_qc_sparse(cuot_ctx *ctx, int64_t n, const uint8_t *a_bits_d, const int64_t *pos_d, int64_t w,
+                   uint8_t *out_bits_d, void *stream);
+
+/* ---- 128-row bit transpose --------------------------------------------------
+ * rows_d[k][m/8] bit m%8 = bit k of block m (bit k = byte k/8, bit k%8), the
+ * order of Mat::bit_transpose (gpu/gpu_matrix.cu:95-121, gpu/gpu_ops.cu:31-70).
+ * N must be a multiple of 32. */
+int cuot_bit_transpose(cuot_ctx *ctx, const cuot_block *in_d, int64_t N, uint8_t *rows_d, void *stream);
+int cuot_bit_untranspose(cuot_ctx *ctx, const uint8_t *rows_d, int64_t N, cuot_block *out_d, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,12 +13,12 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import Block, CuotError, GgmParams, LpnParams, check, load
+from ._lib import Block, CuotError, FerretConfig, GgmParams, LpnParams, SotConfig, check, load
 
 MODE_SOT = 0
 MODE_FERRET = 1
 
-__all__ = ["Context", "Block", "GgmParams", "LpnParams", "CuotError", "MODE_SOT", "MODE_FERRET", "to_block", "load"]
+__all__ = ["Context", "Block", "GgmParams", "LpnParams", "SotConfig", "FerretConfig", "CuotError", "MODE_SOT", "MODE_FERRET", "to_block", "load"]
 
 
 def to_block(b) -> Block:
@@ -119,3 +119,19 @@ class Context:
     def bit_untranspose(self, rows, n_blocks, out, stream=None):
         check(self.lib.cuot_bit_untranspose(self.handle, _ptr(rows), n_blocks, _ptr(out), _stream_ptr(stream)),
               "cuot_bit_untranspose")
+
+    # ---- whole instances --------------------------------------------------------
+    def sot_workspace_bytes(self, cfg: SotConfig) -> int:
+        return int(self.lib.cuot_sot_workspace_bytes(C.byref(cfg)))
+
+    def sot_run(self, cfg: SotConfig, ws, v, w, b, delta=None, stream=None):
+        check(self.lib.cuot_sot_run(self.handle, C.byref(cfg), _ptr(ws), ws.numel() * ws.element_size(), _ptr(v),
+                                    _ptr(w), _ptr(b), _ptr(delta), _stream_ptr(stream)), "cuot_sot_run")
+
+    def ferret_workspace_bytes(self, cfg: FerretConfig) -> int:
+        return int(self.lib.cuot_ferret_workspace_bytes(C.byref(cfg)))
+
+    def ferret_run(self, cfg: FerretConfig, ws, out_s, out_r, delta=None, stream=None):
+        check(self.lib.cuot_ferret_run(self.handle, C.byref(cfg), _ptr(ws), ws.numel() * ws.element_size(),
+                                       _ptr(out_s), _ptr(out_r), _ptr(delta), _stream_ptr(stream)),
+              "cuot_ferret_run")

@@ -31,6 +31,15 @@ class LpnParams(C.Structure):
     _fields_ = [("n", I64), ("k", I64), ("d", I32), ("ntasks", I32), ("seed", Block)]
 
 
+class SotConfig(C.Structure):
+    _fields_ = [("n", I64), ("t", I64), ("h", I32), ("reserved", I32), ("k0", Block), ("k1", Block),
+                ("seed", Block)]
+
+
+class FerretConfig(C.Structure):
+    _fields_ = [("n", I64), ("k", I64), ("t", I64), ("h", I32), ("ntasks", I32), ("seed", Block)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/cuot_amd.h
 SIGNATURES = {
     "cuot_init": (INT, [INT, C.POINTER(P)]),
@@ -50,6 +59,10 @@ SIGNATURES = {
     "cuot_qc_sparse": (INT, [P, I64, P, P, I64, P, P]),
     "cuot_bit_transpose": (INT, [P, P, I64, P, P]),
     "cuot_bit_untranspose": (INT, [P, P, I64, P, P]),
+    "cuot_sot_workspace_bytes": (C.c_size_t, [C.POINTER(SotConfig)]),
+    "cuot_sot_run": (INT, [P, C.POINTER(SotConfig), P, C.c_size_t, P, P, P, P, P]),
+    "cuot_ferret_workspace_bytes": (C.c_size_t, [C.POINTER(FerretConfig)]),
+    "cuot_ferret_run": (INT, [P, C.POINTER(FerretConfig), P, C.c_size_t, P, P, P, P]),
 }
 
 _lib = None

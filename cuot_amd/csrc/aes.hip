@@ -157,3 +157,28 @@ hipError_t cuot_launch_prg(const uint32_t rk[44], uint64_t stream, uint64_t star
   hipLaunchKernelGGL(aes_stream_kernel<true>, dim3(stream_grid(n)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
+
+void cuot_aes_encrypt_host(const uint32_t rk[44], const uint8_t in[16], uint8_t out[16]) {
+  uint8_t S[256];
+  sbox_host(S);
+  const uint8_t* k = (const uint8_t*)rk;
+  uint8_t s[16], u[16];
+  for (int i = 0; i < 16; i++) s[i] = in[i] ^ k[i];
+  for (int r = 1; r <= 10; r++) {
+    for (int c = 0; c < 4; c++)
+      for (int row = 0; row < 4; row++) u[row + 4 * c] = S[s[row + 4 * ((c + row) & 3)]];
+    if (r < 10) {
+      for (int c = 0; c < 4; c++) {
+        const uint8_t a0 = u[4 * c], a1 = u[4 * c + 1], a2 = u[4 * c + 2], a3 = u[4 * c + 3];
+        s[4 * c + 0] = gf8_mul(a0, 2) ^ gf8_mul(a1, 3) ^ a2 ^ a3;
+        s[4 * c + 1] = a0 ^ gf8_mul(a1, 2) ^ gf8_mul(a2, 3) ^ a3;
+        s[4 * c + 2] = a0 ^ a1 ^ gf8_mul(a2, 2) ^ gf8_mul(a3, 3);
+        s[4 * c + 3] = gf8_mul(a0, 3) ^ a1 ^ a2 ^ gf8_mul(a3, 2);
+      }
+    } else {
+      memcpy(s, u, 16);
+    }
+    for (int i = 0; i < 16; i++) s[i] ^= k[16 * r + i];
+  }
+  memcpy(out, s, 16);
+}

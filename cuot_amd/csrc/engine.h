@@ -1,0 +1,38 @@
+// engine.h -- instance plans: workspace carving and stage launch sequences
+// for one SOT or Ferret instance (host side).
+#pragma once
+
+#include "kernels.h"
+
+struct SotPlan {
+  cuot_sot_config cfg;
+  blk *delta, *roots, *alpha;
+  uint8_t* choice;
+  int64_t* pos;
+  uint32_t* a_words;
+  int64_t a_words_total;
+  blk *m0b, *m1b, *sums, *M0, *M1, *csum, *fin;
+  uint32_t* ahat;
+  blk *leaves_s, *leaves_r;
+  char* qc_scratch;
+
+  size_t carve(void* ws);  // returns bytes needed; fills pointers if ws != NULL
+  int validate() const;
+  hipError_t run_setup(hipStream_t st);
+  hipError_t run_prepare(hipStream_t st);
+  hipError_t run_expand(hipStream_t st);
+  hipError_t run_lpn(blk* v, blk* w, uint32_t* b, hipStream_t st);
+};
+
+struct FerretPlan {
+  cuot_ferret_config cfg;
+  blk *delta, *roots, *alpha;
+  uint8_t* choice;
+  blk *sums, *csum, *ssum, *lpn_seed, *K, *Mr, *cbits;
+
+  size_t carve(void* ws);
+  int validate() const;
+  hipError_t run_setup(hipStream_t st);
+  hipError_t run_mpcot(blk* out_s, blk* out_r, hipStream_t st);
+  hipError_t run_lpn(blk* out_s, blk* out_r, hipStream_t st);
+};

@@ -1,0 +1,273 @@
+// engine.hip -- whole SOT and Ferret instances on one GPU, both roles
+// in-process, each stage one launch sequence on the caller's stream.
+//
+// SOT (silent/main.cu:22-72 call order, silent/sender.cu, silent/recver.cu):
+//   setup (seed -> Delta, roots, alpha, a, base-OT messages) -> A^ prepare ->
+//   sender GGM -> masked base-OT rows -> receiver's view -> receiver GGM ->
+//   QC(sender leaves) -> QC(receiver leaves) -> sparse encode of the noise.
+//   Result: v (sender), w and b (receiver) with w_i = v_i ^ b_i * Delta.
+// Ferret (ferret_cot.hpp:79-91 extend, mpcot_reg.h, lpn_f2.h):
+//   setup -> sender MPCOT -> secret sum + receiver's chosen sums -> receiver
+//   MPCOT -> bootstrap COTs (K, M = K ^ c*Delta) -> dual-role LPN.
+#include <string.h>
+
+#include "ctx.h"
+#include "engine.h"
+#include "kernels.h"
+
+namespace {
+
+struct Carve {
+  char* base;
+  size_t off = 0;
+  template <typename T>
+  T* take(size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    T* p = base ? (T*)(base + off) : nullptr;
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// SOT
+// ---------------------------------------------------------------------------
+size_t SotPlan::carve(void* ws) {
+  Carve c{(char*)ws};
+  const int64_t t = cfg.t, h = cfg.h, n = cfg.n;
+  delta = c.take<blk>(1);
+  roots = c.take<blk>(t);
+  alpha = c.take<blk>(t);
+  choice = c.take<uint8_t>(t * h);
+  pos = c.take<int64_t>(t);
+  a_words_total = ((n + 127) / 128) * 4 + 4;  // PRG blocks as words + padding
+  a_words = c.take<uint32_t>(a_words_total);
+  m0b = c.take<blk>(t * h);
+  m1b = c.take<blk>(t * h);
+  sums = c.take<blk>(2 * t * h);
+  M0 = c.take<blk>((h + 1) * t);
+  M1 = c.take<blk>((h + 1) * t);
+  csum = c.take<blk>(t * h);
+  fin = c.take<blk>(t);
+  ahat = c.take<uint32_t>((size_t)cuot_qc_log_len(n) < 63 ? ((size_t)1 << cuot_qc_log_len(n)) : 0);
+  leaves_s = c.take<blk>(2 * n);
+  leaves_r = c.take<blk>(2 * n);
+  qc_scratch = c.take<char>(cuot_qc_scratch_bytes_for(n));
+  return c.off + 256;
+}
+
+int SotPlan::validate() const {
+  CUOT_CHECK_ARG(cfg.n >= 1, "n < 1");
+  CUOT_CHECK_ARG(cfg.h >= 1 && cfg.h <= 16, "h must be in [1, 16]");
+  CUOT_CHECK_ARG(cfg.t >= 1 && cfg.t < (1ll << 31), "t out of range");
+  CUOT_CHECK_ARG((cfg.t << cfg.h) >= 2 * cfg.n, "t * 2^h must cover the 2n leaves");
+  CUOT_CHECK_ARG(cuot_qc_log_len(cfg.n) <= 30, "n too large");
+  return CUOT_OK;
+}
+
+hipError_t SotPlan::run_setup(hipStream_t st) {
+  const int64_t t = cfg.t, h = cfg.h, n = cfg.n;
+  uint32_t rk[44];
+  cuot_aes_expand_host((const uint8_t*)&cfg.seed, rk);
+  hipError_t e;
+  if ((e = cuot_launch_prg(rk, 0, 0, 1, delta, st))) return e;
+  if ((e = cuot_launch_prg(rk, 1, 0, t, roots, st))) return e;
+  if ((e = cuot_launch_prg(rk, 2, 0, t, alpha, st))) return e;
+  if ((e = cuot_launch_prg(rk, 3, 0, (n + 127) / 128, (blk*)a_words, st))) return e;
+  if ((e = cuot_launch_prg(rk, 4, 0, h * t, m0b, st))) return e;
+  if ((e = cuot_launch_prg(rk, 5, 0, h * t, m1b, st))) return e;
+  return cuot_launch_setup_fix(delta, alpha, t, (int)h, choice, pos, a_words, n, a_words_total, st);
+}
+
+hipError_t SotPlan::run_prepare(hipStream_t st) {
+  return cuot_launch_qc_prepare((const uint8_t*)a_words, cfg.n, ahat, qc_scratch, st);
+}
+
+hipError_t SotPlan::run_expand(hipStream_t st) {
+  GgmKArgs a;
+  memset(&a, 0, sizeof(a));
+  a.T0 = cuot_device_T0();
+  cuot_aes_expand_host((const uint8_t*)&cfg.k0, a.rk);
+  cuot_aes_expand_host((const uint8_t*)&cfg.k1, a.rk + 44);
+  a.t = cfg.t;
+  a.h = cfg.h;
+  a.mode = CUOT_MODE_SOT;
+  a.n_out = 2 * cfg.n;
+  a.roots = roots;
+  a.leaves = leaves_s;
+  a.sums = sums;
+  hipError_t e;
+  if ((e = cuot_launch_ggm(a, false, st))) return e;
+  if ((e = cuot_launch_sot_msgs(sums, m0b, m1b, delta, cfg.t, cfg.h, M0, M1, st))) return e;
+  if ((e = cuot_launch_sot_recv_prep(choice, M0, M1, m0b, m1b, cfg.t, cfg.h, csum, fin, st))) return e;
+  a.roots = nullptr;
+  a.sums = nullptr;
+  a.leaves = leaves_r;
+  a.choice = choice;
+  a.csum = csum;
+  a.fin = fin;
+  return cuot_launch_ggm(a, true, st);
+}
+
+hipError_t SotPlan::run_lpn(blk* v, blk* w, uint32_t* b, hipStream_t st) {
+  hipError_t e;
+  if ((e = cuot_launch_qc_encode(ahat, cfg.n, leaves_s, v, qc_scratch, st))) return e;
+  if ((e = cuot_launch_qc_encode(ahat, cfg.n, leaves_r, w, qc_scratch, st))) return e;
+  return cuot_launch_sparse(a_words, cfg.n, pos, (int)cfg.t, b, st);
+}
+
+// ---------------------------------------------------------------------------
+// Ferret
+// ---------------------------------------------------------------------------
+size_t FerretPlan::carve(void* ws) {
+  Carve c{(char*)ws};
+  const int64_t t = cfg.t, h = cfg.h, k = cfg.k;
+  delta = c.take<blk>(1);
+  roots = c.take<blk>(t);
+  alpha = c.take<blk>(t);
+  choice = c.take<uint8_t>(t * h);
+  sums = c.take<blk>(2 * t * h);
+  csum = c.take<blk>(t * h);
+  ssum = c.take<blk>(t);
+  lpn_seed = c.take<blk>(1);
+  K = c.take<blk>(k);
+  Mr = c.take<blk>(k);
+  cbits = c.take<blk>((k + 127) / 128);
+  return c.off + 256;
+}
+
+int FerretPlan::validate() const {
+  CUOT_CHECK_ARG(cfg.n >= 1, "n < 1");
+  CUOT_CHECK_ARG(cfg.h >= 1 && cfg.h <= 16, "h must be in [1, 16]");
+  CUOT_CHECK_ARG(cfg.t >= 1 && cfg.t < (1ll << 31), "t out of range");
+  CUOT_CHECK_ARG((cfg.t << cfg.h) >= cfg.n, "t * 2^h must cover n");
+  CUOT_CHECK_ARG(cfg.k >= 1 && cfg.k < (1ll << 31), "k out of range");
+  CUOT_CHECK_ARG(cfg.ntasks >= 1, "ntasks < 1");
+  return CUOT_OK;
+}
+
+hipError_t FerretPlan::run_setup(hipStream_t st) {
+  uint32_t rk[44];
+  cuot_aes_expand_host((const uint8_t*)&cfg.seed, rk);
+  hipError_t e;
+  if ((e = cuot_launch_prg(rk, 0, 0, 1, delta, st))) return e;
+  if ((e = cuot_launch_prg(rk, 1, 0, cfg.t, roots, st))) return e;
+  if ((e = cuot_launch_prg(rk, 2, 0, cfg.t, alpha, st))) return e;
+  if ((e = cuot_launch_prg(rk, 6, 0, 1, lpn_seed, st))) return e;
+  if ((e = cuot_launch_prg(rk, 7, 0, cfg.k, K, st))) return e;
+  if ((e = cuot_launch_prg(rk, 8, 0, (cfg.k + 127) / 128, cbits, st))) return e;
+  if ((e = cuot_launch_setup_fix(delta, alpha, cfg.t, cfg.h, choice, nullptr, nullptr, 0, 0, st))) return e;
+  return cuot_launch_ferret_boot(K, Mr, (const uint8_t*)cbits, delta, cfg.k, st);
+}
+
+hipError_t FerretPlan::run_mpcot(blk* out_s, blk* out_r, hipStream_t st) {
+  GgmKArgs a;
+  memset(&a, 0, sizeof(a));
+  a.T0 = cuot_device_T0();
+  const cuot_block k0 = {0, 0}, k1 = {1, 0};  // TwoKeyPRP(zero_block, makeBlock(0, 1))
+  cuot_aes_expand_host((const uint8_t*)&k0, a.rk);
+  cuot_aes_expand_host((const uint8_t*)&k1, a.rk + 44);
+  a.t = cfg.t;
+  a.h = cfg.h;
+  a.mode = CUOT_MODE_FERRET;
+  a.n_out = cfg.n;
+  a.roots = roots;
+  a.leaves = out_s;
+  a.sums = sums;
+  hipError_t e;
+  if ((e = cuot_launch_ggm(a, false, st))) return e;
+  if ((e = cuot_launch_ferret_sums(sums, delta, choice, cfg.t, cfg.h, ssum, csum, nullptr, nullptr, st))) return e;
+  a.roots = nullptr;
+  a.sums = nullptr;
+  a.leaves = out_r;
+  a.choice = choice;
+  a.csum = csum;
+  a.fin = ssum;
+  return cuot_launch_ggm(a, true, st);
+}
+
+// The LPN key is the seed block; the host needs it for the AES schedule, so
+// the seed is derived on the host side from the instance seed (one AES).
+hipError_t FerretPlan::run_lpn(blk* out_s, blk* out_r, hipStream_t st) {
+  uint32_t rk_seed[44];
+  cuot_aes_expand_host((const uint8_t*)&cfg.seed, rk_seed);
+  // LPN seed = AES_seed({0, 6}) computed on the host (identical to stream 6)
+  uint8_t ctr[16] = {0};
+  ctr[8] = 6;
+  uint8_t seed6[16];
+  cuot_aes_encrypt_host(rk_seed, ctr, seed6);
+  LpnLaunch l;
+  memset(&l, 0, sizeof(l));
+  cuot_aes_expand_host(seed6, l.rk);
+  l.n = cfg.n;
+  l.k = cfg.k;
+  l.ntasks = cfg.ntasks;
+  l.nroles = 2;
+  l.noise[0] = out_s;
+  l.kk[0] = K;
+  l.out[0] = out_s;
+  l.noise[1] = out_r;
+  l.kk[1] = Mr;
+  l.out[1] = out_r;
+  return cuot_launch_lpn(l, st);
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" size_t cuot_sot_workspace_bytes(const cuot_sot_config* cfg) {
+  if (!cfg) return 0;
+  SotPlan p;
+  p.cfg = *cfg;
+  if (p.validate() != CUOT_OK) return 0;
+  return p.carve(nullptr);
+}
+
+extern "C" int cuot_sot_run(cuot_ctx* ctx, const cuot_sot_config* cfg, void* ws_d, size_t ws_bytes, cuot_block* v_d,
+                            cuot_block* w_d, uint8_t* b_d, cuot_block* delta_d, void* stream) {
+  CUOT_CHECK_ARG(ctx && cfg, "ctx/cfg NULL");
+  SotPlan p;
+  p.cfg = *cfg;
+  int rc = p.validate();
+  if (rc) return rc;
+  CUOT_CHECK_ARG(ws_d && v_d && w_d && b_d, "buffer is NULL");
+  CUOT_CHECK_ARG(ws_bytes >= p.carve(nullptr), "workspace too small (cuot_sot_workspace_bytes)");
+  p.carve(ws_d);
+  CUOT_HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->st(stream);
+  CUOT_HIP_TRY(p.run_setup(st));
+  CUOT_HIP_TRY(p.run_prepare(st));
+  CUOT_HIP_TRY(p.run_expand(st));
+  CUOT_HIP_TRY(p.run_lpn((blk*)v_d, (blk*)w_d, (uint32_t*)b_d, st));
+  if (delta_d) CUOT_HIP_TRY(hipMemcpyAsync(delta_d, p.delta, 16, hipMemcpyDeviceToDevice, st));
+  return CUOT_OK;
+}
+
+extern "C" size_t cuot_ferret_workspace_bytes(const cuot_ferret_config* cfg) {
+  if (!cfg) return 0;
+  FerretPlan p;
+  p.cfg = *cfg;
+  if (p.validate() != CUOT_OK) return 0;
+  return p.carve(nullptr);
+}
+
+extern "C" int cuot_ferret_run(cuot_ctx* ctx, const cuot_ferret_config* cfg, void* ws_d, size_t ws_bytes,
+                               cuot_block* out_s_d, cuot_block* out_r_d, cuot_block* delta_d, void* stream) {
+  CUOT_CHECK_ARG(ctx && cfg, "ctx/cfg NULL");
+  FerretPlan p;
+  p.cfg = *cfg;
+  int rc = p.validate();
+  if (rc) return rc;
+  CUOT_CHECK_ARG(ws_d && out_s_d && out_r_d, "buffer is NULL");
+  CUOT_CHECK_ARG(ws_bytes >= p.carve(nullptr), "workspace too small (cuot_ferret_workspace_bytes)");
+  p.carve(ws_d);
+  CUOT_HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->st(stream);
+  CUOT_HIP_TRY(p.run_setup(st));
+  CUOT_HIP_TRY(p.run_mpcot((blk*)out_s_d, (blk*)out_r_d, st));
+  CUOT_HIP_TRY(p.run_lpn((blk*)out_s_d, (blk*)out_r_d, st));
+  if (delta_d) CUOT_HIP_TRY(hipMemcpyAsync(delta_d, p.delta, 16, hipMemcpyDeviceToDevice, st));
+  return CUOT_OK;
+}

@@ -54,3 +54,19 @@ hipError_t cuot_launch_qc_encode(const uint32_t* ahat, int64_t n, const void* x,
 hipError_t cuot_launch_sparse(const uint32_t* a_words, int64_t n, const int64_t* pos, int w, uint32_t* out,
                               hipStream_t st);
 hipError_t cuot_launch_transpose(const void* in, int64_t N, void* out, bool inverse, hipStream_t st);
+
+// Protocol glue (proto.hip).
+hipError_t cuot_launch_setup_fix(blk* delta, const blk* alpha_blocks, int64_t t, int h, uint8_t* choice,
+                                 int64_t* pos, uint32_t* a_words, int64_t n, int64_t a_words_total,
+                                 hipStream_t st);
+hipError_t cuot_launch_sot_msgs(const blk* sums, const blk* m0b, const blk* m1b, const blk* delta, int64_t t,
+                                int h, blk* M0, blk* M1, hipStream_t st);
+hipError_t cuot_launch_sot_recv_prep(const uint8_t* choice, const blk* M0, const blk* M1, const blk* m0b,
+                                     const blk* m1b, int64_t t, int h, blk* csum, blk* fin, hipStream_t st);
+hipError_t cuot_launch_ferret_sums(const blk* sums, const blk* delta, const uint8_t* choice, int64_t t, int h,
+                                   blk* ssum, blk* csum, blk* lsum, blk* rsum, hipStream_t st);
+hipError_t cuot_launch_ferret_boot(blk* K, blk* Mr, const uint8_t* cbits, const blk* delta, int64_t k,
+                                   hipStream_t st);
+
+// Host single-block AES-128 (aes.hip), rk from cuot_aes_expand_host.
+void cuot_aes_encrypt_host(const uint32_t rk[44], const uint8_t in[16], uint8_t out[16]);

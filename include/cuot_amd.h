@@ -136,6 +136,49 @@ int cuot_qc_sparse(cuot_ctx *ctx, int64_t n, const uint8_t *a_bits_d, const int6
 int cuot_bit_transpose(cuot_ctx *ctx, const cuot_block *in_d, int64_t N, uint8_t *rows_d, void *stream);
 int cuot_bit_untranspose(cuot_ctx *ctx, const uint8_t *rows_d, int64_t N, cuot_block *out_d, void *stream);
 
+/* ---- whole instances (both roles in-process) --------------------------------
+ * Everything derives from `seed` by the counter-mode PRG (stream ids:
+ * 0 Delta (LSB forced 1), 1 roots, 2 punctured indices, 3 circulant a,
+ * 4/5 base-OT messages, 6 Ferret LPN seed, 7/8 Ferret bootstrap K and
+ * choice bits).  The workspace is caller-owned device memory of at least
+ * *_workspace_bytes(cfg) bytes (0 = invalid config).
+ *
+ * SOT: n COTs from 2n = t*2^h GGM leaves (last tree truncated), QC 2n -> n.
+ * Mirrors SOTSender/SOTRecver::{base_ot, seed_expand, dual_lpn}
+ * (silent/roles.h:53-85) with base OTs simulated from the seed.
+ * Out: v_d (sender, n blocks), w_d (receiver, n blocks), b_d (receiver choice
+ * bits, (n+31)/32 words), delta_d (1 block, may be NULL):
+ * w_i = v_i ^ b_i * Delta for every i. */
+typedef struct {
+  int64_t n;
+  int64_t t;
+  int32_t h;
+  int32_t reserved;
+  cuot_block k0, k1; /* GGM PRG keys (SilentConfig leftKey/rightKey) */
+  cuot_block seed;
+} cuot_sot_config;
+
+size_t cuot_sot_workspace_bytes(const cuot_sot_config *cfg);
+int cuot_sot_run(cuot_ctx *ctx, const cuot_sot_config *cfg, void *ws_d, size_t ws_bytes, cuot_block *v_d,
+                 cuot_block *w_d, uint8_t *b_d, cuot_block *delta_d, void *stream);
+
+/* Ferret: MPCOT over t trees of 2^h leaves truncated to n, then the 10-local
+ * LPN with k bootstrap COTs (FerretCOT::extend, ferret_cot.hpp:79-91, with
+ * emp-ot's TwoKeyPRP keys).  Out: out_s_d, out_r_d (n blocks each);
+ * out_r = out_s ^ lsb(out_r) * Delta. */
+typedef struct {
+  int64_t n;
+  int64_t k;
+  int64_t t;
+  int32_t h;
+  int32_t ntasks;
+  cuot_block seed;
+} cuot_ferret_config;
+
+size_t cuot_ferret_workspace_bytes(const cuot_ferret_config *cfg);
+int cuot_ferret_run(cuot_ctx *ctx, const cuot_ferret_config *cfg, void *ws_d, size_t ws_bytes,
+                    cuot_block *out_s_d, cuot_block *out_r_d, cuot_block *delta_d, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -662,3 +662,119 @@ void or_ferret_lpn_indices(int64_t n, int64_t k, orb seed, int d, int ntasks, ui
   lpn_ctx c = {NULL, NULL, idx, d};
   lpn_drive(n, k, seed, d, ntasks, &c, emit_idx, 1);
 }
+
+/* ======================================================================== */
+/* Whole instances from one seed.                                            */
+/* ======================================================================== */
+static orb derive_delta(const orb *seed) {
+  orb d;
+  or_prg(seed, 0, 0, 1, &d);
+  d.lo |= 1;
+  return d;
+}
+
+static uint8_t *derive_choices_th(const orb *seed, int64_t t, int h, int64_t *pos_out) {
+  /* returns c[t][h]; pos_out[t] = t * 2^h + alpha_t */
+  orb *al = (orb *)malloc((size_t)t * sizeof(orb));
+  or_prg(seed, 2, 0, (size_t)t, al);
+  uint8_t *c = (uint8_t *)malloc((size_t)(t * h));
+  for (int64_t tr = 0; tr < t; tr++) {
+    uint32_t alpha = (uint32_t)al[tr].lo & (uint32_t)((1ull << h) - 1);
+    for (int d = 0; d < h; d++) c[tr * h + d] = (uint8_t)(1 - ((alpha >> (h - 1 - d)) & 1));
+    if (pos_out) pos_out[tr] = (tr << h) + alpha;
+  }
+  free(al);
+  return c;
+}
+
+void or_sot_run(const or_sot_cfg *cfg, int nthreads, orb *v, orb *w, uint8_t *b, orb *delta, orb *leaves_s,
+                orb *leaves_r) {
+  const int64_t n = cfg->n, t = cfg->t, N = 2 * n;
+  const int h = cfg->h;
+  uint8_t rk0[176], rk1[176];
+  or_aes_expand(&cfg->k0, rk0);
+  or_aes_expand(&cfg->k1, rk1);
+  orb D = derive_delta(&cfg->seed);
+  orb *roots = (orb *)malloc((size_t)t * sizeof(orb));
+  or_prg(&cfg->seed, 1, 0, (size_t)t, roots);
+  int64_t *pos = (int64_t *)malloc((size_t)t * sizeof(int64_t));
+  uint8_t *cth = derive_choices_th(&cfg->seed, t, h, pos);
+  uint8_t *c = (uint8_t *)malloc((size_t)(t * h)); /* [h][t] layout of recver.cu */
+  for (int64_t tr = 0; tr < t; tr++)
+    for (int d = 0; d < h; d++) c[d * t + tr] = cth[tr * h + d];
+  int64_t ablocks = (n + 127) / 128;
+  orb *ab = (orb *)malloc((size_t)ablocks * sizeof(orb));
+  or_prg(&cfg->seed, 3, 0, (size_t)ablocks, ab);
+  uint8_t *a = (uint8_t *)calloc((size_t)((n + 7) / 8), 1);
+  memcpy(a, ab, (size_t)((n + 7) / 8));
+  if (n % 8) a[(n - 1) / 8] &= (uint8_t)((1u << (n % 8)) - 1);
+  orb *m0b = (orb *)malloc((size_t)(h * t) * sizeof(orb));
+  orb *m1b = (orb *)malloc((size_t)(h * t) * sizeof(orb));
+  or_prg(&cfg->seed, 4, 0, (size_t)(h * t), m0b);
+  or_prg(&cfg->seed, 5, 0, (size_t)(h * t), m1b);
+  orb *M0 = (orb *)malloc((size_t)((h + 1) * t) * sizeof(orb));
+  orb *M1 = (orb *)malloc((size_t)((h + 1) * t) * sizeof(orb));
+  orb *ls = leaves_s ? leaves_s : (orb *)malloc((size_t)N * sizeof(orb));
+  orb *lr = leaves_r ? leaves_r : (orb *)malloc((size_t)N * sizeof(orb));
+  or_sot_sender_expand(rk0, rk1, roots, t, h, D, m0b, m1b, N, ls, M0, M1, nthreads);
+  or_sot_recver_expand(rk0, rk1, t, h, c, M0, M1, m0b, m1b, N, lr, nthreads);
+  or_qc_encode_fast(ls, a, n, v, nthreads);
+  or_qc_encode_fast(lr, a, n, w, nthreads);
+  or_qc_sparse(pos, t, a, n, b);
+  *delta = D;
+  free(roots);
+  free(pos);
+  free(cth);
+  free(c);
+  free(ab);
+  free(a);
+  free(m0b);
+  free(m1b);
+  free(M0);
+  free(M1);
+  if (!leaves_s) free(ls);
+  if (!leaves_r) free(lr);
+}
+
+void or_ferret_run(const or_ferret_cfg *cfg, int nthreads, orb *out_s, orb *out_r, orb *delta) {
+  const int64_t n = cfg->n, t = cfg->t, k = cfg->k;
+  const int h = cfg->h;
+  orb D = derive_delta(&cfg->seed);
+  orb *roots = (orb *)malloc((size_t)t * sizeof(orb));
+  or_prg(&cfg->seed, 1, 0, (size_t)t, roots);
+  uint8_t *cth = derive_choices_th(&cfg->seed, t, h, NULL);
+  orb *ls = (orb *)malloc((size_t)(t * h) * sizeof(orb));
+  orb *rs = (orb *)malloc((size_t)(t * h) * sizeof(orb));
+  orb *ss = (orb *)malloc((size_t)t * sizeof(orb));
+  or_ferret_mpcot_sender(roots, t, h, D, n, out_s, ls, rs, ss, nthreads);
+  orb *csum = (orb *)malloc((size_t)(t * h) * sizeof(orb));
+  for (int64_t i = 0; i < t * h; i++) csum[i] = cth[i] ? rs[i] : ls[i];
+  or_ferret_mpcot_recver(t, h, cth, csum, ss, n, out_r, nthreads);
+  orb seed_lpn;
+  or_prg(&cfg->seed, 6, 0, 1, &seed_lpn);
+  orb *K = (orb *)malloc((size_t)k * sizeof(orb));
+  orb *Mb = (orb *)malloc((size_t)k * sizeof(orb));
+  or_prg(&cfg->seed, 7, 0, (size_t)k, K);
+  int64_t cb = (k + 127) / 128;
+  orb *cbits = (orb *)malloc((size_t)cb * sizeof(orb));
+  or_prg(&cfg->seed, 8, 0, (size_t)cb, cbits);
+  for (int64_t j = 0; j < k; j++) {
+    K[j].lo &= ~1ull;
+    const uint8_t *cbb = (const uint8_t *)cbits;
+    int cj = (cbb[j >> 3] >> (j & 7)) & 1;
+    Mb[j] = K[j];
+    if (cj) Mb[j] = bx(Mb[j], D);
+  }
+  or_ferret_lpn(out_s, n, K, k, seed_lpn, 10, cfg->ntasks, nthreads);
+  or_ferret_lpn(out_r, n, Mb, k, seed_lpn, 10, cfg->ntasks, nthreads);
+  *delta = D;
+  free(roots);
+  free(cth);
+  free(ls);
+  free(rs);
+  free(ss);
+  free(csum);
+  free(K);
+  free(Mb);
+  free(cbits);
+}

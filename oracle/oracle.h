@@ -108,6 +108,43 @@ void or_ferret_lpn(orb *nn, int64_t n, const orb *kk, int64_t k, orb seed, int d
 /* indices only (for tests): idx[i*d + j] */
 void or_ferret_lpn_indices(int64_t n, int64_t k, orb seed, int d, int ntasks, uint32_t *idx);
 
+/* ---- whole instances from one seed (workload definition) ----------------
+ * Seed derivation, stream ids of or_prg(S, stream, index):
+ *   0: Delta = block 0 with bit 0 set (ferret_cot.hpp:27-31 forces LSB 1)
+ *   1: GGM roots, one per tree
+ *   2: punctured index alpha_t = (low 32 bits of block t) mod 2^h
+ *   3: a (QC circulant), bit i = bit i%128 of block i/128, i < n
+ *   4, 5: base-OT messages m0b[d][t], m1b[d][t] (index d*t_count + t)
+ *   6: Ferret LPN seed (block 0)
+ *   7: Ferret bootstrap K_j with bit 0 cleared (base_cot.h:53-58)
+ *   8: Ferret bootstrap choice c_j = bit j%128 of block j/128;
+ *      receiver's M_j = K_j ^ c_j * Delta (LSB = c_j, base_cot.h:59-70)
+ * The receiver's choice at level d is c = 1 - bit (h-1-d) of alpha
+ * (silent/recver.cu:288-296, spcot_recver.h:33-41). */
+typedef struct {
+  int64_t n; /* COTs out; leaves = 2n = t * 2^h (last tree truncated) */
+  int64_t t;
+  int h;
+  orb k0, k1;
+  orb seed;
+} or_sot_cfg;
+
+/* v, w: n blocks; b: (n+7)/8 bytes; delta: 1 block.  Also returns the raw
+ * leaves if leaves_s/leaves_r are non-NULL (2n blocks each). */
+void or_sot_run(const or_sot_cfg *cfg, int nthreads, orb *v, orb *w, uint8_t *b, orb *delta, orb *leaves_s,
+                orb *leaves_r);
+
+typedef struct {
+  int64_t n, k;
+  int64_t t;
+  int h;
+  int ntasks;
+  orb seed;
+} or_ferret_cfg;
+
+/* out_s, out_r: n blocks; delta: 1 block */
+void or_ferret_run(const or_ferret_cfg *cfg, int nthreads, orb *out_s, orb *out_r, orb *delta);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,14 @@ class ORB(C.Structure):
     _fields_ = [("lo", C.c_uint64), ("hi", C.c_uint64)]
 
 
+class SOTCFG(C.Structure):
+    _fields_ = [("n", I64), ("t", I64), ("h", INT), ("k0", ORB), ("k1", ORB), ("seed", ORB)]
+
+
+class FERRETCFG(C.Structure):
+    _fields_ = [("n", I64), ("k", I64), ("t", I64), ("h", INT), ("ntasks", INT), ("seed", ORB)]
+
+
 def build(force: bool = False) -> str:
     if force or not os.path.exists(_LIB_PATH):
         subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
@@ -58,6 +66,8 @@ def lib():
         L.or_bit_untranspose.argtypes = [P, I64, P]
         L.or_ferret_lpn.argtypes = [P, I64, P, I64, ORB, INT, INT, INT]
         L.or_ferret_lpn_indices.argtypes = [I64, I64, ORB, INT, INT, P]
+        L.or_sot_run.argtypes = [C.POINTER(SOTCFG), INT, P, P, P, P, P, P]
+        L.or_ferret_run.argtypes = [C.POINTER(FERRETCFG), INT, P, P, P]
         _lib = L
     return _lib
 
@@ -228,3 +238,24 @@ def ferret_lpn_indices(n, k, seed, d=10, ntasks=1):
     idx = np.zeros(n * d, dtype=np.uint32)
     lib().or_ferret_lpn_indices(n, k, as_orb(seed), d, ntasks, _p(idx))
     return idx.reshape(n, d)
+
+
+# ---- whole instances -----------------------------------------------------------
+def sot_run(n, t, h, k0, k1, seed, nthreads=1, want_leaves=False):
+    cfg = SOTCFG(n, t, h, as_orb(k0), as_orb(k1), as_orb(seed))
+    v, w, d = blocks(n), blocks(n), blocks(1)
+    b = np.zeros((n + 7) // 8, dtype=np.uint8)
+    ls = blocks(2 * n) if want_leaves else None
+    lr = blocks(2 * n) if want_leaves else None
+    lib().or_sot_run(C.byref(cfg), nthreads, _p(v), _p(w), _p(b), _p(d), _p(ls) if want_leaves else None,
+                     _p(lr) if want_leaves else None)
+    if want_leaves:
+        return v, w, b, d[0], ls, lr
+    return v, w, b, d[0]
+
+
+def ferret_run(n, k, t, h, seed, ntasks=1, nthreads=1):
+    cfg = FERRETCFG(n, k, t, h, ntasks, as_orb(seed))
+    s, r, d = blocks(n), blocks(n), blocks(1)
+    lib().or_ferret_run(C.byref(cfg), nthreads, _p(s), _p(r), _p(d))
+    return s, r, d[0]

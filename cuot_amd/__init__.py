@@ -13,12 +13,13 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import Block, CuotError, FerretConfig, GgmParams, LpnParams, SotConfig, check, load
+from ._lib import Block, CuotError, FerretConfig, GgmParams, Instance, LpnParams, SotConfig, check, load
 
 MODE_SOT = 0
 MODE_FERRET = 1
 
-__all__ = ["Context", "Block", "GgmParams", "LpnParams", "SotConfig", "FerretConfig", "CuotError", "MODE_SOT", "MODE_FERRET", "to_block", "load"]
+__all__ = ["Context", "Batch", "Block", "GgmParams", "LpnParams", "SotConfig", "FerretConfig", "Instance",
+           "CuotError", "MODE_SOT", "MODE_FERRET", "BATCH_GRAPH", "to_block", "load"]
 
 
 def to_block(b) -> Block:
@@ -135,3 +136,84 @@ class Context:
         check(self.lib.cuot_ferret_run(self.handle, C.byref(cfg), _ptr(ws), ws.numel() * ws.element_size(),
                                        _ptr(out_s), _ptr(out_r), _ptr(delta), _stream_ptr(stream)),
               "cuot_ferret_run")
+
+    def sot_run_stages(self, cfg: SotConfig, ws, v, w, b, delta, stages: int, stream=None):
+        check(self.lib.cuot_sot_run_stages(self.handle, C.byref(cfg), _ptr(ws), ws.numel() * ws.element_size(),
+                                           _ptr(v), _ptr(w), _ptr(b), _ptr(delta), stages, _stream_ptr(stream)),
+              "cuot_sot_run_stages")
+
+    def ferret_run_stages(self, cfg: FerretConfig, ws, out_s, out_r, delta, stages: int, stream=None):
+        check(self.lib.cuot_ferret_run_stages(self.handle, C.byref(cfg), _ptr(ws), ws.numel() * ws.element_size(),
+                                              _ptr(out_s), _ptr(out_r), _ptr(delta), stages, _stream_ptr(stream)),
+              "cuot_ferret_run_stages")
+
+
+BATCH_STAGES = [("setup", 1), ("qc_prepare", 2), ("ggm_sender", 4), ("ggm_recver", 8), ("qc", 16), ("sparse", 32),
+                ("lpn", 64)]
+BATCH_ALL = 127
+BATCH_GRAPH = 1 << 16
+
+
+class Batch:
+    """Many SOT / Ferret instances as one launch sequence (cuot_batch_*).
+
+    ``instances`` holds ``Instance`` records; the workspace and the outputs are
+    torch device tensors owned by this object: ``out0[i]`` (SOT v / Ferret
+    sender), ``out1[i]`` (SOT w / Ferret receiver), ``bits[i]`` (SOT choice
+    bits, int32 words) and ``delta`` (one block per instance).
+    """
+
+    def __init__(self, ctx: Context, instances, qc_chunk_bytes: int = 0):
+        import torch
+
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.instances = list(instances)
+        arr = (Instance * len(self.instances))(*self.instances)
+        h = C.c_void_p()
+        check(self.lib.cuot_batch_new(ctx.handle, arr, len(self.instances), qc_chunk_bytes, C.byref(h)),
+              "cuot_batch_new")
+        self.handle = h
+        dev = torch.device("cuda", ctx.device)
+        self.ws_bytes = int(self.lib.cuot_batch_workspace_bytes(h))
+        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=dev)
+        self.out0, self.out1, self.bits = [], [], []
+        for ins in self.instances:
+            self.out0.append(torch.empty((ins.n, 2), dtype=torch.int64, device=dev))
+            self.out1.append(torch.empty((ins.n, 2), dtype=torch.int64, device=dev))
+            self.bits.append(torch.empty(((ins.n + 31) // 32,), dtype=torch.int32, device=dev)
+                             if ins.kind == MODE_SOT else None)
+        self.delta = torch.empty((len(self.instances), 2), dtype=torch.int64, device=dev)
+        n = len(self.instances)
+        o0 = (C.c_void_p * n)(*[t.data_ptr() for t in self.out0])
+        o1 = (C.c_void_p * n)(*[t.data_ptr() for t in self.out1])
+        bb = (C.c_void_p * n)(*[(t.data_ptr() if t is not None else None) for t in self.bits])
+        check(self.lib.cuot_batch_bind(h, self.ws.data_ptr(), self.ws_bytes, o0, o1, bb, self.delta.data_ptr()),
+              "cuot_batch_bind")
+
+    def run(self, stages: int = BATCH_ALL, graph: bool = False, stream=None):
+        flags = stages | (BATCH_GRAPH if graph else 0)
+        check(self.lib.cuot_batch_run(self.handle, flags, _stream_ptr(stream)), "cuot_batch_run")
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.cuot_batch_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_instance_struct(kind: int, n: int, t: int, h: int, k: int, seed, ntasks: int = 1, k0=(3242342, 0),
+                         k1=(8993849, 0)) -> Instance:
+    """Instance record (SOT keys default to the SilentConfig leftKey/rightKey)."""
+    return Instance(kind=kind, h=h, n=n, t=t, k=k, ntasks=ntasks, reserved=0, k0=to_block(k0), k1=to_block(k1),
+                    seed=to_block(seed))
+
+
+SOT_STAGES = [("setup", 1), ("qc_prepare", 2), ("ggm_sender", 4), ("ggm_recver", 8), ("qc_sender", 16),
+              ("qc_recver", 32), ("sparse", 64)]
+FERRET_STAGES = [("setup", 1), ("ggm_sender", 2), ("ggm_recver", 4), ("lpn", 8)]

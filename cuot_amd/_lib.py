@@ -40,6 +40,16 @@ class FerretConfig(C.Structure):
     _fields_ = [("n", I64), ("k", I64), ("t", I64), ("h", I32), ("ntasks", I32), ("seed", Block)]
 
 
+class SilentConfig(C.Structure):
+    _fields_ = [("id", I32), ("logOT", I32), ("nTree", C.c_uint64), ("leftKey", C.c_uint32 * 4),
+                ("rightKey", C.c_uint32 * 4), ("gpuPerParty", I32), ("reserved", I32)]
+
+
+class Instance(C.Structure):
+    _fields_ = [("kind", I32), ("h", I32), ("n", I64), ("t", I64), ("k", I64), ("ntasks", I32), ("reserved", I32),
+                ("k0", Block), ("k1", Block), ("seed", Block)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/cuot_amd.h
 SIGNATURES = {
     "cuot_init": (INT, [INT, C.POINTER(P)]),
@@ -61,8 +71,27 @@ SIGNATURES = {
     "cuot_bit_untranspose": (INT, [P, P, I64, P, P]),
     "cuot_sot_workspace_bytes": (C.c_size_t, [C.POINTER(SotConfig)]),
     "cuot_sot_run": (INT, [P, C.POINTER(SotConfig), P, C.c_size_t, P, P, P, P, P]),
+    "cuot_sot_run_stages": (INT, [P, C.POINTER(SotConfig), P, C.c_size_t, P, P, P, P, C.c_uint32, P]),
     "cuot_ferret_workspace_bytes": (C.c_size_t, [C.POINTER(FerretConfig)]),
+    "cuot_ferret_run_stages": (INT, [P, C.POINTER(FerretConfig), P, C.c_size_t, P, P, P, C.c_uint32, P]),
+    "cuot_setdev": (INT, [INT]),
+    "cuot_mpcot_sender": (INT, [P, P, P, P, P, P, P, P, INT, P, INT]),
+    "cuot_mpcot_recver": (INT, [P, P, P, P, P, P, P, INT, P, INT]),
+    "cuot_primal_lpn": (INT, [P, INT, P, I64, P, INT, P, P, P, P, P, INT]),
+    "cuot_sot_sender_new": (INT, [P, C.POINTER(SilentConfig), C.POINTER(Block), C.POINTER(P)]),
+    "cuot_sot_recver_new": (INT, [P, C.POINTER(SilentConfig), C.POINTER(Block), C.POINTER(P)]),
+    "cuot_sot_role_free": (INT, [P]),
+    "cuot_sot_base_ot": (INT, [P]),
+    "cuot_sot_seed_expand": (INT, [P]),
+    "cuot_sot_get_punc_key": (INT, [P, P]),
+    "cuot_sot_dual_lpn": (INT, [P]),
+    "cuot_sot_outputs": (INT, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(I64)]),
     "cuot_ferret_run": (INT, [P, C.POINTER(FerretConfig), P, C.c_size_t, P, P, P, P]),
+    "cuot_batch_new": (INT, [P, C.POINTER(Instance), INT, C.c_size_t, C.POINTER(P)]),
+    "cuot_batch_workspace_bytes": (C.c_size_t, [P]),
+    "cuot_batch_bind": (INT, [P, P, C.c_size_t, C.POINTER(P), C.POINTER(P), C.POINTER(P), P]),
+    "cuot_batch_run": (INT, [P, C.c_uint32, P]),
+    "cuot_batch_free": (INT, [P]),
 }
 
 _lib = None

@@ -73,10 +73,15 @@ CUOT_DEV uint32_t transpose32(uint32_t x, int lane) {
 }
 
 // blocks -> rows.  WG: 256 lanes, 1024 blocks (32 tiles of 32 blocks).
-// rows: 128 rows of N bits, row stride N/32 words.
+// rows: 128 rows of N bits, row stride N/32 words.  blockIdx.y selects a plane
+// (input plane stride in_ps blocks, output plane stride out_ps words).
 __global__ __launch_bounds__(256) void transpose_fwd(const uint4* __restrict__ in, int64_t N,
-                                                     uint32_t* __restrict__ rows) {
+                                                     uint32_t* __restrict__ rows, int P, int64_t in_is, int64_t in_ps,
+                                                     int64_t out_is, int64_t out_ps) {
   __shared__ uint32_t sm[128][33];
+  const int inst = (int)blockIdx.y / P, pl = (int)blockIdx.y % P;
+  in += inst * in_is + pl * in_ps;
+  rows += inst * out_is + pl * out_ps;
   const int lane = threadIdx.x & 31;
   const int grp = threadIdx.x >> 5;  // 8 groups of 32 lanes
   const int64_t ntile = N / 32;
@@ -104,8 +109,12 @@ __global__ __launch_bounds__(256) void transpose_fwd(const uint4* __restrict__ i
 
 // rows -> blocks
 __global__ __launch_bounds__(256) void transpose_inv(const uint32_t* __restrict__ rows, int64_t N,
-                                                     uint4* __restrict__ out) {
+                                                     uint4* __restrict__ out, int P, int64_t in_is, int64_t in_ps,
+                                                     int64_t out_is, int64_t out_ps) {
   __shared__ uint32_t sm[128][33];
+  const int inst = (int)blockIdx.y / P, pl = (int)blockIdx.y % P;
+  rows += inst * in_is + pl * in_ps;
+  out += inst * out_is + pl * out_ps;
   const int lane = threadIdx.x & 31;
   const int grp = threadIdx.x >> 5;
   const int64_t ntile = N / 32;
@@ -147,14 +156,31 @@ hipError_t cuot_launch_sparse(const uint32_t* a_words, int64_t n, const int64_t*
   return hipGetLastError();
 }
 
-hipError_t cuot_launch_transpose(const void* in, int64_t N, void* out, bool inverse, hipStream_t st) {
+hipError_t cuot_launch_transpose_planes2(const void* in, int64_t N, void* out, bool inverse, int planes, int P,
+                                        int64_t in_is, int64_t in_ps, int64_t out_is, int64_t out_ps,
+                                        hipStream_t st) {
+  if (planes < 1 || P < 1 || planes > 65535) return hipErrorInvalidValue;
   int64_t ntile = N / 32;
   int64_t g = (ntile + 31) / 32;
-  if (g > 8192) g = 8192;
+  int64_t cap = planes > 1 ? 16384 / planes : 8192;
+  if (cap < 1) cap = 1;
+  if (g > cap) g = cap;
   if (g < 1) g = 1;
+  dim3 grid((unsigned)g, (unsigned)planes);
   if (!inverse)
-    hipLaunchKernelGGL(transpose_fwd, dim3((unsigned)g), dim3(256), 0, st, (const uint4*)in, N, (uint32_t*)out);
+    hipLaunchKernelGGL(transpose_fwd, grid, dim3(256), 0, st, (const uint4*)in, N, (uint32_t*)out, P, in_is, in_ps,
+                       out_is, out_ps);
   else
-    hipLaunchKernelGGL(transpose_inv, dim3((unsigned)g), dim3(256), 0, st, (const uint32_t*)in, N, (uint4*)out);
+    hipLaunchKernelGGL(transpose_inv, grid, dim3(256), 0, st, (const uint32_t*)in, N, (uint4*)out, P, in_is, in_ps,
+                       out_is, out_ps);
   return hipGetLastError();
+}
+
+hipError_t cuot_launch_transpose_planes(const void* in, int64_t N, void* out, bool inverse, int planes,
+                                       int64_t in_ps, int64_t out_ps, hipStream_t st) {
+  return cuot_launch_transpose_planes2(in, N, out, inverse, planes, planes, 0, in_ps, 0, out_ps, st);
+}
+
+hipError_t cuot_launch_transpose(const void* in, int64_t N, void* out, bool inverse, hipStream_t st) {
+  return cuot_launch_transpose_planes(in, N, out, inverse, 1, 0, 0, st);
 }

@@ -15,11 +15,16 @@ struct SotPlan {
   uint32_t* ahat;
   blk *leaves_s, *leaves_r;
   char* qc_scratch;
+  bool shared_qc = false;  // batch: the QC scratch lives outside the instance
 
   size_t carve(void* ws);  // returns bytes needed; fills pointers if ws != NULL
   int validate() const;
   hipError_t run_setup(hipStream_t st);
   hipError_t run_prepare(hipStream_t st);
+  void ggm_args(GgmKArgs& a) const;
+  hipError_t run_expand_sender(hipStream_t st);
+  hipError_t run_sender_msgs(hipStream_t st);  // masked rows + receiver's view
+  hipError_t run_expand_recver(hipStream_t st);
   hipError_t run_expand(hipStream_t st);
   hipError_t run_lpn(blk* v, blk* w, uint32_t* b, hipStream_t st);
 };
@@ -33,6 +38,11 @@ struct FerretPlan {
   size_t carve(void* ws);
   int validate() const;
   hipError_t run_setup(hipStream_t st);
+  void ggm_args(GgmKArgs& a) const;
+  hipError_t run_mpcot_sender(blk* out_s, hipStream_t st);
+  hipError_t run_sender_sums(hipStream_t st);  // secret sum + receiver's chosen sums
+  hipError_t run_mpcot_recver(blk* out_r, hipStream_t st);
   hipError_t run_mpcot(blk* out_s, blk* out_r, hipStream_t st);
+  void lpn_launch(blk* out_s, blk* out_r, LpnLaunch& l) const;
   hipError_t run_lpn(blk* out_s, blk* out_r, hipStream_t st);
 };

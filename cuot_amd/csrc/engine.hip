@@ -54,7 +54,7 @@ size_t SotPlan::carve(void* ws) {
   ahat = c.take<uint32_t>((size_t)cuot_qc_log_len(n) < 63 ? ((size_t)1 << cuot_qc_log_len(n)) : 0);
   leaves_s = c.take<blk>(2 * n);
   leaves_r = c.take<blk>(2 * n);
-  qc_scratch = c.take<char>(cuot_qc_scratch_bytes_for(n));
+  qc_scratch = shared_qc ? nullptr : c.take<char>(cuot_qc_scratch_bytes_for(n));
   return c.off + 256;
 }
 
@@ -85,8 +85,7 @@ hipError_t SotPlan::run_prepare(hipStream_t st) {
   return cuot_launch_qc_prepare((const uint8_t*)a_words, cfg.n, ahat, qc_scratch, st);
 }
 
-hipError_t SotPlan::run_expand(hipStream_t st) {
-  GgmKArgs a;
+void SotPlan::ggm_args(GgmKArgs& a) const {
   memset(&a, 0, sizeof(a));
   a.T0 = cuot_device_T0();
   cuot_aes_expand_host((const uint8_t*)&cfg.k0, a.rk);
@@ -95,20 +94,38 @@ hipError_t SotPlan::run_expand(hipStream_t st) {
   a.h = cfg.h;
   a.mode = CUOT_MODE_SOT;
   a.n_out = 2 * cfg.n;
+}
+
+hipError_t SotPlan::run_expand_sender(hipStream_t st) {
+  GgmKArgs a;
+  ggm_args(a);
   a.roots = roots;
   a.leaves = leaves_s;
   a.sums = sums;
   hipError_t e;
   if ((e = cuot_launch_ggm(a, false, st))) return e;
+  return run_sender_msgs(st);
+}
+
+hipError_t SotPlan::run_sender_msgs(hipStream_t st) {
+  hipError_t e;
   if ((e = cuot_launch_sot_msgs(sums, m0b, m1b, delta, cfg.t, cfg.h, M0, M1, st))) return e;
-  if ((e = cuot_launch_sot_recv_prep(choice, M0, M1, m0b, m1b, cfg.t, cfg.h, csum, fin, st))) return e;
-  a.roots = nullptr;
-  a.sums = nullptr;
+  return cuot_launch_sot_recv_prep(choice, M0, M1, m0b, m1b, cfg.t, cfg.h, csum, fin, st);
+}
+
+hipError_t SotPlan::run_expand_recver(hipStream_t st) {
+  GgmKArgs a;
+  ggm_args(a);
   a.leaves = leaves_r;
   a.choice = choice;
   a.csum = csum;
   a.fin = fin;
   return cuot_launch_ggm(a, true, st);
+}
+
+hipError_t SotPlan::run_expand(hipStream_t st) {
+  hipError_t e = run_expand_sender(st);
+  return e ? e : run_expand_recver(st);
 }
 
 hipError_t SotPlan::run_lpn(blk* v, blk* w, uint32_t* b, hipStream_t st) {
@@ -162,8 +179,7 @@ hipError_t FerretPlan::run_setup(hipStream_t st) {
   return cuot_launch_ferret_boot(K, Mr, (const uint8_t*)cbits, delta, cfg.k, st);
 }
 
-hipError_t FerretPlan::run_mpcot(blk* out_s, blk* out_r, hipStream_t st) {
-  GgmKArgs a;
+void FerretPlan::ggm_args(GgmKArgs& a) const {
   memset(&a, 0, sizeof(a));
   a.T0 = cuot_device_T0();
   const cuot_block k0 = {0, 0}, k1 = {1, 0};  // TwoKeyPRP(zero_block, makeBlock(0, 1))
@@ -173,14 +189,26 @@ hipError_t FerretPlan::run_mpcot(blk* out_s, blk* out_r, hipStream_t st) {
   a.h = cfg.h;
   a.mode = CUOT_MODE_FERRET;
   a.n_out = cfg.n;
+}
+
+hipError_t FerretPlan::run_mpcot_sender(blk* out_s, hipStream_t st) {
+  GgmKArgs a;
+  ggm_args(a);
   a.roots = roots;
   a.leaves = out_s;
   a.sums = sums;
   hipError_t e;
   if ((e = cuot_launch_ggm(a, false, st))) return e;
-  if ((e = cuot_launch_ferret_sums(sums, delta, choice, cfg.t, cfg.h, ssum, csum, nullptr, nullptr, st))) return e;
-  a.roots = nullptr;
-  a.sums = nullptr;
+  return run_sender_sums(st);
+}
+
+hipError_t FerretPlan::run_sender_sums(hipStream_t st) {
+  return cuot_launch_ferret_sums(sums, delta, choice, cfg.t, cfg.h, ssum, csum, nullptr, nullptr, st);
+}
+
+hipError_t FerretPlan::run_mpcot_recver(blk* out_r, hipStream_t st) {
+  GgmKArgs a;
+  ggm_args(a);
   a.leaves = out_r;
   a.choice = choice;
   a.csum = csum;
@@ -188,9 +216,14 @@ hipError_t FerretPlan::run_mpcot(blk* out_s, blk* out_r, hipStream_t st) {
   return cuot_launch_ggm(a, true, st);
 }
 
+hipError_t FerretPlan::run_mpcot(blk* out_s, blk* out_r, hipStream_t st) {
+  hipError_t e = run_mpcot_sender(out_s, st);
+  return e ? e : run_mpcot_recver(out_r, st);
+}
+
 // The LPN key is the seed block; the host needs it for the AES schedule, so
 // the seed is derived on the host side from the instance seed (one AES).
-hipError_t FerretPlan::run_lpn(blk* out_s, blk* out_r, hipStream_t st) {
+void FerretPlan::lpn_launch(blk* out_s, blk* out_r, LpnLaunch& l) const {
   uint32_t rk_seed[44];
   cuot_aes_expand_host((const uint8_t*)&cfg.seed, rk_seed);
   // LPN seed = AES_seed({0, 6}) computed on the host (identical to stream 6)
@@ -198,7 +231,6 @@ hipError_t FerretPlan::run_lpn(blk* out_s, blk* out_r, hipStream_t st) {
   ctr[8] = 6;
   uint8_t seed6[16];
   cuot_aes_encrypt_host(rk_seed, ctr, seed6);
-  LpnLaunch l;
   memset(&l, 0, sizeof(l));
   cuot_aes_expand_host(seed6, l.rk);
   l.n = cfg.n;
@@ -211,6 +243,11 @@ hipError_t FerretPlan::run_lpn(blk* out_s, blk* out_r, hipStream_t st) {
   l.noise[1] = out_r;
   l.kk[1] = Mr;
   l.out[1] = out_r;
+}
+
+hipError_t FerretPlan::run_lpn(blk* out_s, blk* out_r, hipStream_t st) {
+  LpnLaunch l;
+  lpn_launch(out_s, out_r, l);
   return cuot_launch_lpn(l, st);
 }
 
@@ -225,8 +262,9 @@ extern "C" size_t cuot_sot_workspace_bytes(const cuot_sot_config* cfg) {
   return p.carve(nullptr);
 }
 
-extern "C" int cuot_sot_run(cuot_ctx* ctx, const cuot_sot_config* cfg, void* ws_d, size_t ws_bytes, cuot_block* v_d,
-                            cuot_block* w_d, uint8_t* b_d, cuot_block* delta_d, void* stream) {
+extern "C" int cuot_sot_run_stages(cuot_ctx* ctx, const cuot_sot_config* cfg, void* ws_d, size_t ws_bytes,
+                                   cuot_block* v_d, cuot_block* w_d, uint8_t* b_d, cuot_block* delta_d,
+                                   uint32_t stages, void* stream) {
   CUOT_CHECK_ARG(ctx && cfg, "ctx/cfg NULL");
   SotPlan p;
   p.cfg = *cfg;
@@ -237,12 +275,24 @@ extern "C" int cuot_sot_run(cuot_ctx* ctx, const cuot_sot_config* cfg, void* ws_
   p.carve(ws_d);
   CUOT_HIP_TRY(hipSetDevice(ctx->device));
   hipStream_t st = ctx->st(stream);
-  CUOT_HIP_TRY(p.run_setup(st));
-  CUOT_HIP_TRY(p.run_prepare(st));
-  CUOT_HIP_TRY(p.run_expand(st));
-  CUOT_HIP_TRY(p.run_lpn((blk*)v_d, (blk*)w_d, (uint32_t*)b_d, st));
-  if (delta_d) CUOT_HIP_TRY(hipMemcpyAsync(delta_d, p.delta, 16, hipMemcpyDeviceToDevice, st));
+  if (stages & CUOT_SOT_STAGE_SETUP) CUOT_HIP_TRY(p.run_setup(st));
+  if (stages & CUOT_SOT_STAGE_QC_PREPARE) CUOT_HIP_TRY(p.run_prepare(st));
+  if (stages & CUOT_SOT_STAGE_GGM_SENDER) CUOT_HIP_TRY(p.run_expand_sender(st));
+  if (stages & CUOT_SOT_STAGE_GGM_RECVER) CUOT_HIP_TRY(p.run_expand_recver(st));
+  if (stages & CUOT_SOT_STAGE_QC_SENDER)
+    CUOT_HIP_TRY(cuot_launch_qc_encode(p.ahat, p.cfg.n, p.leaves_s, (blk*)v_d, p.qc_scratch, st));
+  if (stages & CUOT_SOT_STAGE_QC_RECVER)
+    CUOT_HIP_TRY(cuot_launch_qc_encode(p.ahat, p.cfg.n, p.leaves_r, (blk*)w_d, p.qc_scratch, st));
+  if (stages & CUOT_SOT_STAGE_SPARSE)
+    CUOT_HIP_TRY(cuot_launch_sparse(p.a_words, p.cfg.n, p.pos, (int)p.cfg.t, (uint32_t*)b_d, st));
+  if (delta_d && (stages & CUOT_SOT_STAGE_SETUP))
+    CUOT_HIP_TRY(hipMemcpyAsync(delta_d, p.delta, 16, hipMemcpyDeviceToDevice, st));
   return CUOT_OK;
+}
+
+extern "C" int cuot_sot_run(cuot_ctx* ctx, const cuot_sot_config* cfg, void* ws_d, size_t ws_bytes, cuot_block* v_d,
+                            cuot_block* w_d, uint8_t* b_d, cuot_block* delta_d, void* stream) {
+  return cuot_sot_run_stages(ctx, cfg, ws_d, ws_bytes, v_d, w_d, b_d, delta_d, CUOT_SOT_STAGE_ALL, stream);
 }
 
 extern "C" size_t cuot_ferret_workspace_bytes(const cuot_ferret_config* cfg) {
@@ -253,8 +303,9 @@ extern "C" size_t cuot_ferret_workspace_bytes(const cuot_ferret_config* cfg) {
   return p.carve(nullptr);
 }
 
-extern "C" int cuot_ferret_run(cuot_ctx* ctx, const cuot_ferret_config* cfg, void* ws_d, size_t ws_bytes,
-                               cuot_block* out_s_d, cuot_block* out_r_d, cuot_block* delta_d, void* stream) {
+extern "C" int cuot_ferret_run_stages(cuot_ctx* ctx, const cuot_ferret_config* cfg, void* ws_d, size_t ws_bytes,
+                                      cuot_block* out_s_d, cuot_block* out_r_d, cuot_block* delta_d,
+                                      uint32_t stages, void* stream) {
   CUOT_CHECK_ARG(ctx && cfg, "ctx/cfg NULL");
   FerretPlan p;
   p.cfg = *cfg;
@@ -265,9 +316,16 @@ extern "C" int cuot_ferret_run(cuot_ctx* ctx, const cuot_ferret_config* cfg, voi
   p.carve(ws_d);
   CUOT_HIP_TRY(hipSetDevice(ctx->device));
   hipStream_t st = ctx->st(stream);
-  CUOT_HIP_TRY(p.run_setup(st));
-  CUOT_HIP_TRY(p.run_mpcot((blk*)out_s_d, (blk*)out_r_d, st));
-  CUOT_HIP_TRY(p.run_lpn((blk*)out_s_d, (blk*)out_r_d, st));
-  if (delta_d) CUOT_HIP_TRY(hipMemcpyAsync(delta_d, p.delta, 16, hipMemcpyDeviceToDevice, st));
+  if (stages & CUOT_FERRET_STAGE_SETUP) CUOT_HIP_TRY(p.run_setup(st));
+  if (stages & CUOT_FERRET_STAGE_GGM_SENDER) CUOT_HIP_TRY(p.run_mpcot_sender((blk*)out_s_d, st));
+  if (stages & CUOT_FERRET_STAGE_GGM_RECVER) CUOT_HIP_TRY(p.run_mpcot_recver((blk*)out_r_d, st));
+  if (stages & CUOT_FERRET_STAGE_LPN) CUOT_HIP_TRY(p.run_lpn((blk*)out_s_d, (blk*)out_r_d, st));
+  if (delta_d && (stages & CUOT_FERRET_STAGE_SETUP))
+    CUOT_HIP_TRY(hipMemcpyAsync(delta_d, p.delta, 16, hipMemcpyDeviceToDevice, st));
   return CUOT_OK;
+}
+
+extern "C" int cuot_ferret_run(cuot_ctx* ctx, const cuot_ferret_config* cfg, void* ws_d, size_t ws_bytes,
+                               cuot_block* out_s_d, cuot_block* out_r_d, cuot_block* delta_d, void* stream) {
+  return cuot_ferret_run_stages(ctx, cfg, ws_d, ws_bytes, out_s_d, out_r_d, delta_d, CUOT_FERRET_STAGE_ALL, stream);
 }

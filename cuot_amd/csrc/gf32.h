@@ -66,36 +66,42 @@ static inline int gf32_cantor_basis(uint32_t beta[32]) {
 
 #ifdef __HIPCC__
 // ---- device -------------------------------------------------------------------
-// y = c * x for 32 bitsliced rows.  UNIFORM: c is the same in every lane of the
-// wave (branches on its bits are scalar); otherwise per-lane bit masks.
+// y ^= c * x for 32 bitsliced rows (Horner form: w runs through x * t^i; a
+// multiplication by t is a rotation of the 32 planes, which full unrolling
+// turns into register renaming, plus the reduction t^32 = t^7 + t^3 + t^2 + 1
+// of the plane that wraps).  UNIFORM: c is the same in every lane of the wave
+// (branches on its bits are scalar); otherwise per-lane bit masks.
 template <bool UNIFORM>
-__device__ __forceinline__ void gf32_mulc(const uint32_t* __restrict__ x, uint32_t c, uint32_t* __restrict__ y) {
-  uint32_t p[63];
+__device__ __forceinline__ void gf32_mulc_acc(const uint32_t* __restrict__ x, uint32_t c, uint32_t* __restrict__ y) {
+  uint32_t w[32];
 #pragma unroll
-  for (int i = 0; i < 63; i++) p[i] = 0;
+  for (int j = 0; j < 32; j++) w[j] = x[j];
 #pragma unroll
   for (int i = 0; i < 32; i++) {
     if (UNIFORM) {
       if ((c >> i) & 1u) {
 #pragma unroll
-        for (int j = 0; j < 32; j++) p[i + j] ^= x[j];
+        for (int j = 0; j < 32; j++) y[j] ^= w[(j - i) & 31];
       }
     } else {
       const uint32_t m = 0u - ((c >> i) & 1u);
 #pragma unroll
-      for (int j = 0; j < 32; j++) p[i + j] ^= x[j] & m;
+      for (int j = 0; j < 32; j++) y[j] ^= w[(j - i) & 31] & m;
     }
+    // w <- w * t: physical slot (31 - i) & 31 holds the plane that wraps to
+    // t^0; fold it into t^2, t^3, t^7 (slots relative to the rotation).
+    const uint32_t top = w[(31 - i) & 31];
+    w[(2 - i - 1) & 31] ^= top;
+    w[(3 - i - 1) & 31] ^= top;
+    w[(7 - i - 1) & 31] ^= top;
   }
-  // reduce t^(32+i) = t^(i+7) + t^(i+3) + t^(i+2) + t^i, from the top
+}
+
+// y = c * x
+template <bool UNIFORM>
+__device__ __forceinline__ void gf32_mulc(const uint32_t* __restrict__ x, uint32_t c, uint32_t* __restrict__ y) {
 #pragma unroll
-  for (int k = 62; k >= 32; k--) {
-    const uint32_t v = p[k];
-    p[k - 32 + 7] ^= v;
-    p[k - 32 + 3] ^= v;
-    p[k - 32 + 2] ^= v;
-    p[k - 32] ^= v;
-  }
-#pragma unroll
-  for (int i = 0; i < 32; i++) y[i] = p[i];
+  for (int j = 0; j < 32; j++) y[j] = 0;
+  gf32_mulc_acc<UNIFORM>(x, c, y);
 }
 #endif

@@ -186,10 +186,8 @@ CUOT_DEV void dfs_subtree(const AesLds& aes, const uint32_t* rk, blk root, bool 
 }
 
 template <int L, bool RECV>
-__global__ __launch_bounds__(WG) void ggm_kernel(GgmKArgs a) {
-  __shared__ Lds s;
+CUOT_DEV void ggm_tree(const GgmKArgs& a, Lds& s, int64_t tree) {
   const int tid = threadIdx.x;
-  const int64_t tree = blockIdx.x;
   aes_stage_table(s.aes, a.T0, tid, WG);
   if (tid == 0) s.nodes[0] = RECV ? bzero() : a.roots[tree];
   __syncthreads();
@@ -255,9 +253,35 @@ __global__ __launch_bounds__(WG) void ggm_kernel(GgmKArgs a) {
   }
 }
 
+template <int L, bool RECV>
+__global__ __launch_bounds__(WG) void ggm_kernel(GgmKArgs a) {
+  __shared__ Lds s;
+  ggm_tree<L, RECV>(a, s, blockIdx.x);
+}
+
+// Batch: one WG per tree of every instance in the table; start[i] is the first
+// global tree of instance i (start[ninst] = total).  All instances share L.
+template <int L, bool RECV>
+__global__ __launch_bounds__(WG) void ggm_batch_kernel(const GgmKArgs* __restrict__ tab,
+                                                       const int64_t* __restrict__ start, int ninst) {
+  __shared__ Lds s;
+  const int64_t g = blockIdx.x;
+  int lo = 0, hi = ninst - 1;  // last i with start[i] <= g
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  ggm_tree<L, RECV>(tab[lo], s, g - start[lo]);
+}
+
+int ggm_split(int h) { return h <= 8 ? 0 : (h - 8 < LMAX ? h - 8 : LMAX); }
+
 template <bool RECV>
 hipError_t launch_ggm(const GgmKArgs& a, hipStream_t st) {
-  const int L = a.h <= 8 ? 0 : (a.h - 8 < LMAX ? a.h - 8 : LMAX);
+  const int L = ggm_split(a.h);
   dim3 grid((unsigned)a.t), block(WG);
   switch (L) {
 #define CUOT_GGM_CASE(X) \
@@ -278,8 +302,40 @@ hipError_t launch_ggm(const GgmKArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
+template <bool RECV>
+hipError_t launch_ggm_batch(const GgmKArgs* tab, const int64_t* start, int ninst, int64_t total, int L,
+                            hipStream_t st) {
+  dim3 grid((unsigned)total), block(WG);
+  switch (L) {
+#define CUOT_GGM_CASE(X) \
+  case X:                \
+    hipLaunchKernelGGL((ggm_batch_kernel<X, RECV>), grid, block, 0, st, tab, start, ninst); \
+    break;
+    CUOT_GGM_CASE(0)
+    CUOT_GGM_CASE(1)
+    CUOT_GGM_CASE(2)
+    CUOT_GGM_CASE(3)
+    CUOT_GGM_CASE(4)
+    CUOT_GGM_CASE(5)
+    CUOT_GGM_CASE(6)
+#undef CUOT_GGM_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 }  // namespace
+
+int cuot_ggm_split(int h) { return ggm_split(h); }
 
 hipError_t cuot_launch_ggm(const GgmKArgs& a, bool receiver, hipStream_t st) {
   return receiver ? launch_ggm<true>(a, st) : launch_ggm<false>(a, st);
+}
+
+hipError_t cuot_launch_ggm_batch(const GgmKArgs* tab_d, const int64_t* start_d, int ninst, int64_t total_trees,
+                                 int L, bool receiver, hipStream_t st) {
+  if (ninst < 1 || total_trees < 1 || total_trees > 0x7fffffff) return hipErrorInvalidValue;
+  return receiver ? launch_ggm_batch<true>(tab_d, start_d, ninst, total_trees, L, st)
+                  : launch_ggm_batch<false>(tab_d, start_d, ninst, total_trees, L, st);
 }

@@ -19,6 +19,12 @@ struct GgmKArgs {
   const blk* fin;        // receiver: [t] SOT final-row value / Ferret secret sum
 };
 hipError_t cuot_launch_ggm(const GgmKArgs& a, bool receiver, hipStream_t st);
+// depth handled per lane below the shared top levels (trees of depth h)
+int cuot_ggm_split(int h);
+// batch of instances with the same cuot_ggm_split(h) == L and role; tab_d and
+// start_d (ninst + 1 prefix of tree counts) in device memory
+hipError_t cuot_launch_ggm_batch(const GgmKArgs* tab_d, const int64_t* start_d, int ninst, int64_t total_trees,
+                                 int L, bool receiver, hipStream_t st);
 
 // AES-128 ECB / CTR streams.  T0: device AES table (cuot_device_T0()).
 const uint32_t* cuot_device_T0();
@@ -42,6 +48,13 @@ struct LpnLaunch {
   blk* out[2];
 };
 hipError_t cuot_launch_lpn(const LpnLaunch& l, hipStream_t st);
+// batch form: per-instance argument records (cuot_lpn_args_bytes each, filled
+// by cuot_lpn_args_fill, which returns the instance's WG count or -1) and the
+// prefix of WG counts, both in device memory; all instances share nroles
+size_t cuot_lpn_args_bytes();
+int64_t cuot_lpn_args_fill(const LpnLaunch& l, void* out);
+hipError_t cuot_launch_lpn_batch(const void* tabs_d, const int64_t* wg_start_d, int ninst, int64_t total_wgs,
+                                 int nroles, hipStream_t st);
 
 // Quasi-cyclic encoder (qc.hip).
 int cuot_qc_log_len(int64_t n);
@@ -49,11 +62,36 @@ size_t cuot_qc_scratch_bytes_for(int64_t n);
 hipError_t cuot_launch_qc_prepare(const uint8_t* a_bits, int64_t n, uint32_t* ahat, void* scratch, hipStream_t st);
 hipError_t cuot_launch_qc_encode(const uint32_t* ahat, int64_t n, const void* x, void* y, void* scratch,
                                  hipStream_t st);
+// Batches of instances with the same transform length 2^m (one launch per
+// step, blockIdx.y = instance).  Tables live in device memory.
+struct QcIo {
+  const blk* x;  // 2n input blocks
+  blk* y;        // n output blocks
+  int64_t n;
+};
+struct QcPrep {
+  const uint8_t* a_bits;  // n bits of the circulant's first column
+  uint32_t* ahat;         // 2^m words
+  int64_t n;
+};
+size_t cuot_qc_batch_scratch_bytes(int m, int B);
+bool cuot_qc_batchable(int m);   // B > 1 allowed for transform length 2^m
+hipError_t cuot_qc_configure();  // one-time device set-up (outside graph capture)
+hipError_t cuot_launch_qc_prepare_batch(const QcPrep* prep_d, int B, int m, hipStream_t st);
+hipError_t cuot_launch_qc_encode_batch(const QcIo* io_d, const uint32_t* const* ahats_d, int B, int m,
+                                       void* scratch, hipStream_t st);
 
 // Sparse encode and bit transpose (bits.hip).
 hipError_t cuot_launch_sparse(const uint32_t* a_words, int64_t n, const int64_t* pos, int w, uint32_t* out,
                               hipStream_t st);
 hipError_t cuot_launch_transpose(const void* in, int64_t N, void* out, bool inverse, hipStream_t st);
+// planes: blockIdx.y; strides in blocks (block side) and 32-bit words (row side)
+hipError_t cuot_launch_transpose_planes(const void* in, int64_t N, void* out, bool inverse, int planes,
+                                       int64_t in_ps, int64_t out_ps, hipStream_t st);
+// planes grouped by instance: y -> (y / P, y % P); offset = inst * is + p * ps
+hipError_t cuot_launch_transpose_planes2(const void* in, int64_t N, void* out, bool inverse, int planes, int P,
+                                        int64_t in_is, int64_t in_ps, int64_t out_is, int64_t out_ps,
+                                        hipStream_t st);
 
 // Protocol glue (proto.hip).
 hipError_t cuot_launch_setup_fix(blk* delta, const blk* alpha_blocks, int64_t t, int h, uint8_t* choice,

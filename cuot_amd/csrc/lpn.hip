@@ -44,73 +44,99 @@ CUOT_DEV void task_shape(int64_t rows, int64_t& g4, int64_t& t1) {
 }
 
 template <int NR>
+CUOT_DEV void lpn_unit(const AesLds& aes, const LpnKArgs& a, int64_t u) {
+  // locate the task and the unit inside it
+  int64_t task = u / a.units_per_task;
+  if (task >= a.ntasks) task = a.ntasks - 1;
+  const int64_t start = task * a.width;
+  const int64_t end = (task == a.ntasks - 1) ? a.n : start + a.width;
+  const int64_t lu = u - task * a.units_per_task;
+  int64_t g4, t1;
+  task_shape(end - start, g4, t1);
+  if (lu < g4) {
+    const int64_t i = start + 4 * lu;
+    uint32_t w[4 * D];
+#pragma unroll
+    for (int m = 0; m < D; m++) {
+      blk c = make_uint4((uint32_t)m, 0u, (uint32_t)i, (uint32_t)((uint64_t)i >> 32));
+      blk e = aes_enc(aes, a.rk, c);
+      w[4 * m + 0] = e.x;
+      w[4 * m + 1] = e.y;
+      w[4 * m + 2] = e.z;
+      w[4 * m + 3] = e.w;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      blk acc[NR];
+#pragma unroll
+      for (int rr = 0; rr < NR; rr++) acc[rr] = a.noise[rr][i + q];
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        uint32_t idx = w[q * D + j] & a.mask;
+        if (idx >= (uint32_t)a.k) idx -= (uint32_t)a.k;
+#pragma unroll
+        for (int rr = 0; rr < NR; rr++) acc[rr] = bxor(acc[rr], a.kk[rr][idx]);
+      }
+#pragma unroll
+      for (int rr = 0; rr < NR; rr++) a.out[rr][i + q] = acc[rr];
+    }
+  } else if (lu - g4 < t1) {
+    const int64_t i = start + 4 * g4 + (lu - g4);
+    uint32_t w[12];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      blk c = make_uint4((uint32_t)m, 0u, (uint32_t)i, (uint32_t)((uint64_t)i >> 32));
+      blk e = aes_enc(aes, a.rk, c);
+      w[4 * m + 0] = e.x;
+      w[4 * m + 1] = e.y;
+      w[4 * m + 2] = e.z;
+      w[4 * m + 3] = e.w;
+    }
+    blk acc[NR];
+#pragma unroll
+    for (int rr = 0; rr < NR; rr++) acc[rr] = a.noise[rr][i];
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      const uint32_t idx = w[j] % (uint32_t)a.k;
+#pragma unroll
+      for (int rr = 0; rr < NR; rr++) acc[rr] = bxor(acc[rr], a.kk[rr][idx]);
+    }
+#pragma unroll
+    for (int rr = 0; rr < NR; rr++) a.out[rr][i] = acc[rr];
+  }
+}
+
+template <int NR>
 __global__ __launch_bounds__(256) void lpn_kernel(LpnKArgs a) {
   __shared__ uint32_t tab[CUOT_AES_LDS_WORDS];
   aes_stage_table(tab, a.T0, threadIdx.x, 256);
   __syncthreads();
   const AesLds aes = aes_lds(tab);
   const int64_t stride = (int64_t)gridDim.x * 256;
-  for (int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x; u < a.units_total; u += stride) {
-    // locate the task and the unit inside it
-    int64_t task = u / a.units_per_task;
-    if (task >= a.ntasks) task = a.ntasks - 1;
-    const int64_t start = task * a.width;
-    const int64_t end = (task == a.ntasks - 1) ? a.n : start + a.width;
-    const int64_t lu = u - task * a.units_per_task;
-    int64_t g4, t1;
-    task_shape(end - start, g4, t1);
-    if (lu < g4) {
-      const int64_t i = start + 4 * lu;
-      uint32_t w[4 * D];
-#pragma unroll
-      for (int m = 0; m < D; m++) {
-        blk c = make_uint4((uint32_t)m, 0u, (uint32_t)i, (uint32_t)((uint64_t)i >> 32));
-        blk e = aes_enc(aes, a.rk, c);
-        w[4 * m + 0] = e.x;
-        w[4 * m + 1] = e.y;
-        w[4 * m + 2] = e.z;
-        w[4 * m + 3] = e.w;
-      }
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        blk acc[NR];
-#pragma unroll
-        for (int rr = 0; rr < NR; rr++) acc[rr] = a.noise[rr][i + q];
-#pragma unroll
-        for (int j = 0; j < D; j++) {
-          uint32_t idx = w[q * D + j] & a.mask;
-          if (idx >= (uint32_t)a.k) idx -= (uint32_t)a.k;
-#pragma unroll
-          for (int rr = 0; rr < NR; rr++) acc[rr] = bxor(acc[rr], a.kk[rr][idx]);
-        }
-#pragma unroll
-        for (int rr = 0; rr < NR; rr++) a.out[rr][i + q] = acc[rr];
-      }
-    } else if (lu - g4 < t1) {
-      const int64_t i = start + 4 * g4 + (lu - g4);
-      uint32_t w[12];
-#pragma unroll
-      for (int m = 0; m < 3; m++) {
-        blk c = make_uint4((uint32_t)m, 0u, (uint32_t)i, (uint32_t)((uint64_t)i >> 32));
-        blk e = aes_enc(aes, a.rk, c);
-        w[4 * m + 0] = e.x;
-        w[4 * m + 1] = e.y;
-        w[4 * m + 2] = e.z;
-        w[4 * m + 3] = e.w;
-      }
-      blk acc[NR];
-#pragma unroll
-      for (int rr = 0; rr < NR; rr++) acc[rr] = a.noise[rr][i];
-#pragma unroll
-      for (int j = 0; j < D; j++) {
-        const uint32_t idx = w[j] % (uint32_t)a.k;
-#pragma unroll
-        for (int rr = 0; rr < NR; rr++) acc[rr] = bxor(acc[rr], a.kk[rr][idx]);
-      }
-#pragma unroll
-      for (int rr = 0; rr < NR; rr++) a.out[rr][i] = acc[rr];
-    }
+  for (int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x; u < a.units_total; u += stride) lpn_unit<NR>(aes, a, u);
+}
+
+// Batch: wg_start[i] = first WG of instance i (prefix of ceil(units / 256));
+// one WG covers 256 units of a single instance, so the round keys stay uniform.
+template <int NR>
+__global__ __launch_bounds__(256) void lpn_batch_kernel(const LpnKArgs* __restrict__ tabs,
+                                                        const int64_t* __restrict__ wg_start, int ninst) {
+  __shared__ uint32_t tab[CUOT_AES_LDS_WORDS];
+  const int64_t g = blockIdx.x;
+  int lo = 0, hi = ninst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (wg_start[mid] <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
   }
+  const LpnKArgs& a = tabs[lo];
+  aes_stage_table(tab, a.T0, threadIdx.x, 256);
+  __syncthreads();
+  const AesLds aes = aes_lds(tab);
+  const int64_t u = (g - wg_start[lo]) * 256 + threadIdx.x;
+  if (u < a.units_total) lpn_unit<NR>(aes, a, u);
 }
 
 int64_t units_of(int64_t rows) {
@@ -120,10 +146,11 @@ int64_t units_of(int64_t rows) {
 
 }  // namespace
 
-hipError_t cuot_launch_lpn(const LpnLaunch& l, hipStream_t st) {
-  if (l.n <= 0) return hipSuccess;
-  LpnKArgs a;
+namespace {
+
+bool fill_args(const LpnLaunch& l, LpnKArgs& a) {
   memset(&a, 0, sizeof(a));
+  if (l.n <= 0 || l.k < 1 || l.nroles < 1 || l.nroles > 2) return false;
   a.T0 = cuot_device_T0();
   memcpy(a.rk, l.rk, sizeof(a.rk));
   a.n = l.n;
@@ -145,6 +172,15 @@ hipError_t cuot_launch_lpn(const LpnLaunch& l, hipStream_t st) {
     a.kk[r] = l.kk[r];
     a.out[r] = l.out[r];
   }
+  return true;
+}
+
+}  // namespace
+
+hipError_t cuot_launch_lpn(const LpnLaunch& l, hipStream_t st) {
+  if (l.n <= 0) return hipSuccess;
+  LpnKArgs a;
+  if (!fill_args(l, a)) return hipErrorInvalidValue;
   int64_t g = (a.units_total + 255) / 256;
   if (g > 8192) g = 8192;
   if (g < 1) g = 1;
@@ -152,5 +188,26 @@ hipError_t cuot_launch_lpn(const LpnLaunch& l, hipStream_t st) {
     hipLaunchKernelGGL(lpn_kernel<2>, dim3((unsigned)g), dim3(256), 0, st, a);
   else
     hipLaunchKernelGGL(lpn_kernel<1>, dim3((unsigned)g), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+size_t cuot_lpn_args_bytes() { return sizeof(LpnKArgs); }
+
+int64_t cuot_lpn_args_fill(const LpnLaunch& l, void* out) {
+  LpnKArgs a;
+  if (!fill_args(l, a)) return -1;
+  memcpy(out, &a, sizeof(a));
+  return (a.units_total + 255) / 256;
+}
+
+hipError_t cuot_launch_lpn_batch(const void* tabs_d, const int64_t* wg_start_d, int ninst, int64_t total_wgs,
+                                 int nroles, hipStream_t st) {
+  if (ninst < 1 || total_wgs < 1 || total_wgs > 0x7fffffff) return hipErrorInvalidValue;
+  if (nroles == 2)
+    hipLaunchKernelGGL(lpn_batch_kernel<2>, dim3((unsigned)total_wgs), dim3(256), 0, st, (const LpnKArgs*)tabs_d,
+                       wg_start_d, ninst);
+  else
+    hipLaunchKernelGGL(lpn_batch_kernel<1>, dim3((unsigned)total_wgs), dim3(256), 0, st, (const LpnKArgs*)tabs_d,
+                       wg_start_d, ninst);
   return hipGetLastError();
 }

@@ -7,32 +7,49 @@
 // Algorithm: binary polynomial product a(x) * e1(x) by an additive FFT over
 // GF(2^32) (Lin-Chung-Han novel-basis FFT on a Cantor basis), then the fold
 // mod x^n - 1.  Bits are packed 16 per field element (chunk polynomials of
-// degree < 16 multiply without reduction, so the chunk products are exact),
-// and the 128 rows are bitsliced: block x_{n+16c+b} IS bit-plane b of chunk c
-// for all 128 rows, so no bit transpose is needed.
+// degree < 16 multiply without reduction, so chunk products are exact), and the
+// 128 rows are bitsliced: block x_{n+16c+b} IS bit-plane b of chunk c for all
+// 128 rows, so no bit transpose is needed.
 //
 // Data layout (device scratch): X[plane][elem] of 16-byte words (128 rows),
-// 32 planes x M elements, M = 2^m >= 2*ceil(n/16) - 1.
+// 32 planes x M elements, M = 2^m >= 2*ceil(n/16) - 1, plus a ping-pong buffer
+// of G planes for the out-of-place conversion levels.
 //
-// Steps (forward, per party):  pack -> basis conversion (division by the
-// subspace polynomials s_k, XOR only) -> duplicate -> FFT layers m-2..0 ->
-// pointwise * A^ -> inverse FFT layers 0..m-1 -> inverse conversion ->
-// overlap-add of 31-bit chunk products, fold mod x^n - 1, xor x_0.
+// Encode (per party):
+//   pack (16 planes, lower half)
+//   basis conversion, levels m-2..B: one out-of-place pass per level, run per
+//     group of G planes so the group stays in the last-level cache;
+//     levels B-1..0 in one LDS pass per contiguous tile of one plane
+//   FFT: LDS radix passes of up to 8 layers (the first pass also performs the
+//     trivial top layer and the duplication of the lower half); the lowest
+//     pass does forward layers, the pointwise product by A^, and the inverse
+//     layers in one LDS round trip
+//   inverse FFT radix passes, inverse conversion (LDS pass, global levels),
+//     and the top inverse level fused into the overlap-add / fold / xor x_0.
 #include <string.h>
 
+#include "conv_bits.h"
 #include "gf32.h"
 #include "kernels.h"
 
 namespace {
 
-// ---- helpers ----------------------------------------------------------------
-struct QcGeo {
-  int m;        // log2 M
-  int64_t M;    // transform length
-  int64_t n;    // code length
+constexpr int CONV_B = 12;     // conversion levels done in LDS (tile 2^B elements of one plane)
+constexpr int CONV_G = 4;      // planes per conversion group (ping-pong buffer size)
+constexpr int FFT_TC = 256;    // elements per FFT LDS tile (2^R * C)
+constexpr int FFT_STRIDE = 132;  // LDS words per element (32 planes x 4 words + pad)
+constexpr int FFT_THREADS = 512;
+// the row-layout conversion handles strings of 2^5 .. 2^20 coefficients (128 KB LDS)
+constexpr int ROW_MP_MIN = 5, ROW_MP_MAX = 20;
+
+struct Beta {
+  uint32_t v[32];
 };
 
-// lower terms of s_k: e = 2^j for j a proper submask of k
+// Cantor basis beta_0..beta_31 (uploaded once per process by configure_lds)
+__constant__ uint32_t c_beta[32];
+
+// lower terms of s_k: e = 2^j for j a proper submask of k (at most 15 for k < 31)
 CUOT_DEV int lower_terms(int k, uint32_t* e) {
   int c = 0;
   for (int j = 0; j < k; j++)
@@ -40,7 +57,7 @@ CUOT_DEV int lower_terms(int k, uint32_t* e) {
   return c;
 }
 
-// C(g) = sum_b g_b * beta_{b+1}  (the twist s_k(shift of chunk g))
+// C(g) = sum_b g_b * beta_{b+1}
 CUOT_DEV uint32_t chunk_const(const uint32_t* beta, uint64_t g) {
   uint32_t c = 0;
   for (int b = 0; g; b++, g >>= 1)
@@ -48,20 +65,24 @@ CUOT_DEV uint32_t chunk_const(const uint32_t* beta, uint64_t g) {
   return c;
 }
 
-struct Beta {
-  uint32_t v[32];
-};
+CUOT_DEV uint4 x4(uint4 a, uint4 b) { return make_uint4(a.x ^ b.x, a.y ^ b.y, a.z ^ b.z, a.w ^ b.w); }
+CUOT_DEV uint4 z4() { return make_uint4(0, 0, 0, 0); }
 
 // ---- pack -------------------------------------------------------------------
-// X[b][c] = x1[16c + b] (zero beyond n), c < M/2, planes 0..15.
-__global__ void qc_pack_x(const uint4* __restrict__ x1, int64_t n, uint4* __restrict__ X, int64_t M) {
+// X[b][c] = x1[16c + b] (zero beyond n), c < M/2, planes 0..15.  blockIdx.y
+// selects the instance of a batch (X stride XS blocks, io table or io1).
+__global__ void qc_pack_x(const QcIo* __restrict__ io, QcIo io1, uint4* __restrict__ X, int64_t M, int64_t XS) {
+  const QcIo q = io ? io[blockIdx.y] : io1;
+  const uint4* x1 = q.x + q.n;
+  const int64_t n = q.n;
+  X += (int64_t)blockIdx.y * XS;
   const int64_t half = M >> 1;
   const int64_t total = half * 16;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t c = t >> 4;
     const int b = (int)(t & 15);
     const int64_t src = 16 * c + b;
-    X[(int64_t)b * M + c] = src < n ? x1[src] : make_uint4(0, 0, 0, 0);
+    X[(int64_t)b * M + c] = src < n ? x1[src] : z4();
   }
 }
 
@@ -78,125 +99,339 @@ __global__ void qc_pack_a(const uint8_t* __restrict__ a, int64_t n, uint4* __res
   }
 }
 
-// ---- basis conversion steps (plane-wise XOR, u32 granularity) ---------------
-// Polynomial of Mp elements (chunks of 2^(k+1)), planes [0, P), plane stride M.
-// hi_fix: hi_i ^= XOR_{e in T_k, e > i} hi_{i + half - e}   for i < e_max
-__global__ void qc_hi_fix(uint32_t* __restrict__ X, int64_t M, int P, int64_t Mp, int k) {
-  uint32_t e[16];
-  const int ne = lower_terms(k, e);
-  if (ne == 0) return;
-  const uint32_t emax = e[ne - 1];
-  const int64_t half = (int64_t)1 << k;
-  const int64_t nchunk = Mp >> (k + 1);
-  const int64_t total = (int64_t)P * nchunk * emax * 4;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-    const int w = (int)(t & 3);
-    int64_t r = t >> 2;
-    const int64_t i = r % emax;
-    r /= emax;
-    const int64_t g = r % nchunk;
-    const int p = (int)(r / nchunk);
-    uint32_t* base = X + ((int64_t)p * M + g * 2 * half + half) * 4 + w;
-    uint32_t v = base[i * 4];
-    for (int q = 0; q < ne; q++)
-      if ((int64_t)e[q] > i) v ^= base[(i + half - e[q]) * 4];
-    base[i * 4] = v;
-  }
-}
+// ---- basis conversion -------------------------------------------------------
+// Division by s_k on chunks of 2^(k+1) (forward, monomial -> novel basis):
+//   q_i = hi_i ^ XOR_{e in T_k, e > i} hi_{i + half - e}
+//   r_j = lo_j ^ XOR_{e in T_k, e <= j} q_{j - e}
+// Multiplication back (inverse): lo_j ^= XOR_{e <= j} hi_{j-e} (original hi),
+// then hi_i ^= XOR_{e > i} hi_{i + half - e}.  Only positions i < e_max of hi
+// change, and they read positions >= e_max (half >= 2 e_max), so each step is
+// a data-parallel pass.
 
-// lo_update: lo_j ^= XOR_{e in T_k, e <= j} hi_{j - e}
-__global__ void qc_lo_update(uint32_t* __restrict__ X, int64_t M, int P, int64_t Mp, int k) {
-  uint32_t e[16];
-  const int ne = lower_terms(k, e);
-  if (ne == 0) return;
-  const int64_t half = (int64_t)1 << k;
-  const int64_t total = (int64_t)P * (Mp >> 1) * 4;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-    const int w = (int)(t & 3);
-    int64_t r = t >> 2;
-    const int64_t half_elems = Mp >> 1;
-    const int64_t li = r % half_elems;  // index among low halves
-    const int p = (int)(r / half_elems);
-    const int64_t g = li >> k, j = li & (half - 1);
-    uint32_t* lo = X + ((int64_t)p * M + g * 2 * half) * 4 + w;
-    const uint32_t* hi = lo + half * 4;
-    uint32_t v = lo[j * 4];
-    for (int q = 0; q < ne; q++)
-      if ((int64_t)e[q] <= j) v ^= hi[(j - e[q]) * 4];
-    lo[j * 4] = v;
-  }
-}
-
-// duplicate the converted lower half into the upper half, zero planes 16..31
-__global__ void qc_dup(uint4* __restrict__ X, int64_t M) {
-  const int64_t half = M >> 1;
-  const int64_t total = 32 * half;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-    const int p = (int)(t / half);
-    const int64_t e = t % half;
-    if (p < 16) {
-      X[(int64_t)p * M + half + e] = X[(int64_t)p * M + e];
-    } else {
-      X[(int64_t)p * M + e] = make_uint4(0, 0, 0, 0);
-      X[(int64_t)p * M + half + e] = make_uint4(0, 0, 0, 0);
-    }
-  }
-}
-
-// ---- FFT layer --------------------------------------------------------------
-// lane = (butterfly b, word w): u = g*2^(k+1) + j, v = u + 2^k, c = C(g).
-// forward: u ^= c*v; v ^= u.   inverse: v ^= u; u ^= c*v.
+// one level, out-of-place, planes [0, P) of a polynomial of Mp elements
 template <bool INV>
-__global__ __launch_bounds__(256) void qc_fft_layer(uint32_t* __restrict__ X, int64_t M, int k, Beta beta) {
-  const int64_t nb = M >> 1;
-  const int64_t total = nb * 4;
+__global__ void qc_conv_level(const uint4* __restrict__ in, uint4* __restrict__ out, int64_t M, int P, int64_t Mp,
+                              int k) {
+  uint32_t e[16];
+  const int ne = lower_terms(k, e);
+  const uint32_t emax = ne ? e[ne - 1] : 0;
   const int64_t half = (int64_t)1 << k;
+  const int64_t total = (int64_t)P * Mp;
+  const int lgMp = 63 - __builtin_clzll((unsigned long long)Mp);
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-    const int w = (int)(t & 3);
-    const int64_t b = t >> 2;
-    const int64_t g = b >> k, j = b & (half - 1);
-    const int64_t u = g * 2 * half + j, v = u + half;
-    const uint32_t c = chunk_const(beta.v, (uint64_t)g);
-    uint32_t xu[32], xv[32], cv[32];
-#pragma unroll
-    for (int p = 0; p < 32; p++) {
-      xu[p] = X[((int64_t)p * M + u) * 4 + w];
-      xv[p] = X[((int64_t)p * M + v) * 4 + w];
-    }
-    if (!INV) {
-      gf32_mulc<false>(xv, c, cv);
-#pragma unroll
-      for (int p = 0; p < 32; p++) {
-        xu[p] ^= cv[p];
-        xv[p] ^= xu[p];
+    const int p = (int)(t >> lgMp);
+    const int64_t i = t & (Mp - 1);
+    const int64_t pos = i & (2 * half - 1);
+    const uint4* lo = in + (int64_t)p * M + (i - pos);
+    const uint4* hi = lo + half;
+    uint4 v;
+    if (pos < half) {
+      const int64_t j = pos;
+      v = lo[j];
+      for (int q = 0; q < ne; q++) {
+        if ((int64_t)e[q] > j) break;
+        const int64_t s = j - e[q];
+        uint4 qs = hi[s];
+        if (!INV && s < (int64_t)emax) {
+          for (int r = 0; r < ne; r++)
+            if ((int64_t)e[r] > s) qs = x4(qs, hi[s + half - e[r]]);
+        }
+        v = x4(v, qs);
       }
     } else {
-#pragma unroll
-      for (int p = 0; p < 32; p++) xv[p] ^= xu[p];
-      gf32_mulc<false>(xv, c, cv);
-#pragma unroll
-      for (int p = 0; p < 32; p++) xu[p] ^= cv[p];
+      const int64_t s = pos - half;
+      v = hi[s];
+      if (s < (int64_t)emax) {
+        for (int r = 0; r < ne; r++)
+          if ((int64_t)e[r] > s) v = x4(v, hi[s + half - e[r]]);
+      }
     }
-#pragma unroll
-    for (int p = 0; p < 32; p++) {
-      X[((int64_t)p * M + u) * 4 + w] = xu[p];
-      X[((int64_t)p * M + v) * 4 + w] = xv[p];
-    }
+    out[(int64_t)p * M + i] = v;
   }
 }
 
-__global__ __launch_bounds__(256) void qc_pointwise(uint32_t* __restrict__ X, int64_t M,
-                                                    const uint32_t* __restrict__ ahat) {
-  const int64_t total = M * 4;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-    const int w = (int)(t & 3);
-    const int64_t e = t >> 2;
+// levels B-1..0 (forward) or 0..B-1 (inverse) in LDS on contiguous tiles of
+// 2^B elements of one plane; reads src, writes dst (may be equal).
+template <bool INV>
+__global__ __launch_bounds__(512) void qc_conv_lds(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t M,
+                                                   int P, int64_t Mp, int B) {
+  extern __shared__ uint4 sm[];
+  const int64_t tile_n = (int64_t)1 << B;
+  const int64_t tiles_pp = Mp >> B;
+  const int64_t ntiles = (int64_t)P * tiles_pp;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int p = (int)(tile / tiles_pp);
+    const int64_t off = (int64_t)p * M + (tile % tiles_pp) * tile_n;
+    for (int64_t i = threadIdx.x; i < tile_n; i += blockDim.x) sm[i] = src[off + i];
+    __syncthreads();
+    for (int step = 0; step < B; step++) {
+      const int k = INV ? step : B - 1 - step;
+      uint32_t e[16];
+      const int ne = lower_terms(k, e);
+      if (ne == 0) continue;
+      const uint32_t emax = e[ne - 1];
+      const int half = 1 << k;
+      for (int phase = 0; phase < 2; phase++) {
+        const bool do_hi = (phase == 0) != INV;  // forward: hi_fix first
+        if (do_hi) {
+          const int64_t cnt = (tile_n >> (k + 1)) * emax;
+          for (int64_t x = threadIdx.x; x < cnt; x += blockDim.x) {
+            const int64_t g = x / emax, i = x % emax;
+            uint4* h = sm + g * 2 * half + half;
+            uint4 v = h[i];
+            for (int r = 0; r < ne; r++)
+              if ((int64_t)e[r] > i) v = x4(v, h[i + half - e[r]]);
+            h[i] = v;
+          }
+        } else {
+          const int64_t cnt = tile_n >> 1;
+          for (int64_t x = threadIdx.x; x < cnt; x += blockDim.x) {
+            const int64_t g = x >> k, j = x & (half - 1);
+            uint4* l = sm + g * 2 * half;
+            const uint4* h = l + half;
+            uint4 v = l[j];
+            for (int r = 0; r < ne; r++) {
+              if ((int64_t)e[r] > j) break;
+              v = x4(v, h[j - e[r]]);
+            }
+            l[j] = v;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int64_t i = threadIdx.x; i < tile_n; i += blockDim.x) dst[off + i] = sm[i];
+    __syncthreads();
+  }
+}
+
+// Row-string conversion: after a 128-row bit transpose of each plane, every
+// (plane, row) is a string of 2^mp coefficients (the bits of one row over
+// the element index); tiles are consecutive strings of tile_words words.
+struct BarrierSync {
+  __device__ void operator()() const { __syncthreads(); }
+};
+
+__global__ __launch_bounds__(1024) void qc_conv_rows(uint32_t* __restrict__ Bw, int64_t tile_words, int64_t ntiles,
+                                                     int64_t tiles_per_inst, int64_t inst_words, int mp,
+                                                     int inverse) {
+  extern __shared__ uint32_t sw[];
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint32_t* g = Bw + (tile / tiles_per_inst) * inst_words + (tile % tiles_per_inst) * tile_words;
+    for (int64_t i = threadIdx.x; i < tile_words; i += blockDim.x) sw[i] = g[i];
+    __syncthreads();
+    cb_convert(sw, mp, inverse != 0, threadIdx.x, blockDim.x, BarrierSync());
+    for (int64_t i = threadIdx.x; i < tile_words; i += blockDim.x) g[i] = sw[i];
+    __syncthreads();
+  }
+}
+
+// ---- FFT radix pass -----------------------------------------------------------
+// Tile: T = 2^R elements spaced 2^lo apart, times C consecutive low columns.
+// Layers k in [klo, khi] (lo <= klo, khi < lo + R).  forward: u ^= c v; v ^= u;
+// inverse: v ^= u; u ^= c v; c = C(idx_u >> (k+1)).
+enum { PASS_FWD = 0, PASS_INV = 1, PASS_FUSED = 2 };
+
+struct FftPassArgs {
+  uint32_t* X;
+  int64_t M;
+  int lo, R, C, lgC;
+  int mode;
+  int khi;           // highest layer to apply (FWD/FUSED: start, INV: end)
+  int first;         // load only planes 0..15 of the lower half (top layer input)
+  const uint32_t* ahat;
+  const uint32_t* const* ahats;  // per-instance A^ (batch) or NULL
+  int64_t XS_words;  // instance stride of X in 32-bit words (blockIdx.y)
+};
+
+__device__ __attribute__((noinline)) void fft_layer(int lo, int R, int C, int lgC, uint32_t* lds, int64_t hib,
+                                                  int64_t cb, int k, bool inv) {
+  const int s = k - lo;
+  const int T = 1 << R;
+  const int ntask = (T >> 1) * C * 4;
+  for (int tau = threadIdx.x; tau < ntask; tau += blockDim.x) {
+    const int w = tau & 3;
+    const int c = (tau >> 2) & (C - 1);
+    const int bp = (tau >> 2) >> lgC;
+    const int tu = ((bp >> s) << (s + 1)) | (bp & ((1 << s) - 1));
+    const int tv = tu | (1 << s);
+    const int64_t idx = (hib << (lo + R)) | ((int64_t)tu << lo) | (cb * C + c);
+    const uint64_t g = (uint64_t)idx >> (k + 1);
+    uint32_t* pu = lds + (tu * C + c) * FFT_STRIDE + w;
+    uint32_t* pv = lds + (tv * C + c) * FFT_STRIDE + w;
+    uint32_t xu[32], xv[32];
+#pragma unroll
+    for (int p = 0; p < 32; p++) {
+      xu[p] = pu[4 * p];
+      xv[p] = pv[4 * p];
+    }
+    const uint64_t g0 = __builtin_amdgcn_readfirstlane((uint32_t)g) | ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(g >> 32)) << 32);
+    const bool uni = __all(g == g0);
+    if (inv) {
+#pragma unroll
+      for (int p = 0; p < 32; p++) xv[p] ^= xu[p];
+    }
+    if (uni) {
+      const uint32_t cst = __builtin_amdgcn_readfirstlane(chunk_const(c_beta, g0));
+      gf32_mulc_acc<true>(xv, cst, xu);
+    } else {
+      gf32_mulc_acc<false>(xv, chunk_const(c_beta, g), xu);
+    }
+    if (!inv) {
+#pragma unroll
+      for (int p = 0; p < 32; p++) xv[p] ^= xu[p];
+    }
+#pragma unroll
+    for (int p = 0; p < 32; p++) {
+      pu[4 * p] = xu[p];
+      pv[4 * p] = xv[p];
+    }
+  }
+  __syncthreads();
+}
+
+// pointwise product by A^ (per-element constant) on the tile
+__device__ __attribute__((noinline)) void fft_pointwise(int lo, int R, int C, int lgC,
+                                                      const uint32_t* __restrict__ ahat, uint32_t* lds, int64_t hib,
+                                                      int64_t cb) {
+  const int TC = (1 << R) * C;
+  for (int tau = threadIdx.x; tau < TC * 4; tau += blockDim.x) {
+    const int w = tau & 3;
+    const int el = tau >> 2;  // = t*C + c
+    const int t = el >> lgC, c = el & (C - 1);
+    const int64_t idx = (hib << (lo + R)) | ((int64_t)t << lo) | (cb * C + c);
+    uint32_t* pe = lds + el * FFT_STRIDE + w;
     uint32_t x[32], y[32];
 #pragma unroll
-    for (int p = 0; p < 32; p++) x[p] = X[((int64_t)p * M + e) * 4 + w];
-    gf32_mulc<false>(x, ahat[e], y);
+    for (int p = 0; p < 32; p++) x[p] = pe[4 * p];
+    gf32_mulc<false>(x, ahat[idx], y);
 #pragma unroll
-    for (int p = 0; p < 32; p++) X[((int64_t)p * M + e) * 4 + w] = y[p];
+    for (int p = 0; p < 32; p++) pe[4 * p] = y[p];
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(FFT_THREADS) void qc_fft_pass(FftPassArgs a) {
+  extern __shared__ uint32_t lds[];
+  a.X += (int64_t)blockIdx.y * a.XS_words;
+  if (a.ahats) a.ahat = a.ahats[blockIdx.y];
+  const int T = 1 << a.R;
+  const int TC = T * a.C;
+  const int64_t cblocks = ((int64_t)1 << a.lo) >> a.lgC;
+  const int64_t ntiles = a.M / TC;
+  const int64_t half = a.M >> 1;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t cb = tile & (cblocks - 1), hib = tile >> (a.lo - a.lgC);
+    // load: f = ((p*T + t)*C + c)*4 + w  (w, c, t fastest: contiguous runs)
+    const int nf = TC * 32 * 4;
+    for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+      const int w = f & 3;
+      const int c = (f >> 2) & (a.C - 1);
+      const int pt = (f >> 2) >> a.lgC;
+      const int t = pt & (T - 1), p = pt >> a.R;
+      const int64_t idx = (hib << (a.lo + a.R)) | ((int64_t)t << a.lo) | (cb * a.C + c);
+      uint32_t v;
+      if (a.first)
+        v = (idx < half && p < 16) ? a.X[((int64_t)p * a.M + idx) * 4 + w] : 0u;
+      else
+        v = a.X[((int64_t)p * a.M + idx) * 4 + w];
+      lds[(t * a.C + c) * FFT_STRIDE + p * 4 + w] = v;
+    }
+    __syncthreads();
+    if (a.mode == PASS_FWD || a.mode == PASS_FUSED) {
+      for (int k = a.khi; k >= a.lo; k--) fft_layer(a.lo, a.R, a.C, a.lgC, lds, hib, cb, k, false);
+    }
+    if (a.mode == PASS_FUSED) fft_pointwise(a.lo, a.R, a.C, a.lgC, a.ahat, lds, hib, cb);
+    if (a.mode == PASS_INV || a.mode == PASS_FUSED) {
+      for (int k = a.lo; k <= a.khi; k++) fft_layer(a.lo, a.R, a.C, a.lgC, lds, hib, cb, k, true);
+    }
+    for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+      const int w = f & 3;
+      const int c = (f >> 2) & (a.C - 1);
+      const int pt = (f >> 2) >> a.lgC;
+      const int t = pt & (T - 1), p = pt >> a.R;
+      const int64_t idx = (hib << (a.lo + a.R)) | ((int64_t)t << a.lo) | (cb * a.C + c);
+      a.X[((int64_t)p * a.M + idx) * 4 + w] = lds[(t * a.C + c) * FFT_STRIDE + p * 4 + w];
+    }
+    __syncthreads();
+  }
+}
+
+
+// ---- scalar transform of the single polynomial a (A^ = its evaluations) ------
+// Same conversion + FFT as the bitsliced path, on one 32-bit field element per
+// index instead of 128 bitsliced rows (A^ is computed once per code).
+CUOT_DEV uint32_t gf32_mul_dev(uint32_t a, uint32_t b) {
+  uint64_t r = 0;
+  for (int i = 0; i < 32; i++)
+    if ((b >> i) & 1u) r ^= (uint64_t)a << i;
+  for (int k = 62; k >= 32; k--)
+    if ((r >> k) & 1ull) r ^= ((uint64_t)CUOT_GF_POLY_LOW | (1ull << 32)) << (k - 32);
+  return (uint32_t)r;
+}
+
+__global__ void qcs_pack(const QcPrep* __restrict__ pr, QcPrep pr1, int64_t M) {
+  const QcPrep q = pr ? pr[blockIdx.y] : pr1;
+  const uint8_t* a = q.a_bits;
+  const int64_t n = q.n;
+  uint32_t* A = q.ahat;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t v = 0;
+    if (c < (M >> 1)) {
+      for (int b = 0; b < 16; b++) {
+        const int64_t src = 16 * c + b;
+        if (src < n) v |= (uint32_t)((a[src >> 3] >> (src & 7)) & 1u) << b;
+      }
+    }
+    A[c] = v;
+  }
+}
+
+__global__ void qcs_hi_fix(const QcPrep* __restrict__ pr, QcPrep pr1, int64_t Mp, int k) {
+  uint32_t* A = (pr ? pr[blockIdx.y] : pr1).ahat;
+  uint32_t e[16];
+  const int ne = lower_terms(k, e);
+  if (ne == 0) return;
+  const int64_t emax = e[ne - 1], half = (int64_t)1 << k;
+  const int64_t total = (Mp >> (k + 1)) * emax;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = t / emax, i = t % emax;
+    uint32_t* h = A + g * 2 * half + half;
+    uint32_t v = h[i];
+    for (int r = 0; r < ne; r++)
+      if ((int64_t)e[r] > i) v ^= h[i + half - e[r]];
+    h[i] = v;
+  }
+}
+
+__global__ void qcs_lo_update(const QcPrep* __restrict__ pr, QcPrep pr1, int64_t Mp, int k) {
+  uint32_t* A = (pr ? pr[blockIdx.y] : pr1).ahat;
+  uint32_t e[16];
+  const int ne = lower_terms(k, e);
+  if (ne == 0) return;
+  const int64_t half = (int64_t)1 << k;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (Mp >> 1); t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = t >> k, j = t & (half - 1);
+    uint32_t* l = A + g * 2 * half;
+    uint32_t v = l[j];
+    for (int r = 0; r < ne; r++) {
+      if ((int64_t)e[r] > j) break;
+      v ^= l[half + j - e[r]];
+    }
+    l[j] = v;
+  }
+}
+
+__global__ void qcs_fft_layer(const QcPrep* __restrict__ pr, QcPrep pr1, int64_t M, int k) {
+  uint32_t* A = (pr ? pr[blockIdx.y] : pr1).ahat;
+  const int64_t half = (int64_t)1 << k;
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < (M >> 1); b += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = b >> k, j = b & (half - 1);
+    const int64_t u = g * 2 * half + j, v = u + half;
+    const uint32_t c = chunk_const(c_beta, (uint64_t)g);
+    const uint32_t xu = A[u] ^ gf32_mul_dev(c, A[v]);
+    A[u] = xu;
+    A[v] = A[v] ^ xu;
   }
 }
 
@@ -209,28 +444,56 @@ __global__ void qc_extract_row0(const uint32_t* __restrict__ X, int64_t M, uint3
   }
 }
 
-// y_i = x0_i ^ H(i) ^ H(i + n), H(s) = plane (s&15) of chunk s>>4 ^ plane
-// (s&15)+16 of chunk (s>>4)-1.
-CUOT_DEV uint4 hcontrib(const uint4* __restrict__ X, int64_t M, int64_t s) {
-  const int64_t c = s >> 4;
-  const int b = (int)(s & 15);
-  uint4 v = make_uint4(0, 0, 0, 0);
-  if (c < M) v = X[(int64_t)b * M + c];
-  if (c >= 1 && c - 1 < M) {
-    uint4 w = X[(int64_t)(b + 16) * M + c - 1];
-    v.x ^= w.x;
-    v.y ^= w.y;
-    v.z ^= w.z;
-    v.w ^= w.w;
+// ---- output: top inverse level + overlap-add + fold + x0 ---------------------
+// H_c (plane p) after the top inverse level k = m-1 on X (r = X[0..M/2), q = X[M/2..M)):
+//   c <  M/2: r_c ^ XOR_{e in T, e <= c} q_{c-e}
+//   c >= M/2: q_s ^ XOR_{e in T, e > s} q_{s + M/2 - e},  s = c - M/2
+struct TopTerms {
+  uint32_t e[16];
+  int ne;
+};
+
+CUOT_DEV uint4 h_top(const uint4* __restrict__ X, int64_t M, int p, int64_t c, const TopTerms& tt) {
+  const int64_t half = M >> 1;
+  const uint4* r = X + (int64_t)p * M;
+  const uint4* q = r + half;
+  uint4 v;
+  if (c < half) {
+    v = r[c];
+    for (int i = 0; i < tt.ne; i++) {
+      if ((int64_t)tt.e[i] > c) break;
+      v = x4(v, q[c - tt.e[i]]);
+    }
+  } else {
+    const int64_t s = c - half;
+    v = q[s];
+    for (int i = 0; i < tt.ne; i++)
+      if ((int64_t)tt.e[i] > s) v = x4(v, q[s + half - tt.e[i]]);
   }
   return v;
 }
 
-__global__ void qc_output(const uint4* __restrict__ X, int64_t M, const uint4* __restrict__ x0, int64_t n,
-                          uint4* __restrict__ y) {
+// H(s) = plane (s & 15) of chunk s>>4  ^  plane (s & 15) + 16 of chunk (s>>4) - 1
+CUOT_DEV uint4 hcontrib(const uint4* __restrict__ X, int64_t M, int64_t s, const TopTerms& tt) {
+  const int64_t c = s >> 4;
+  const int b = (int)(s & 15);
+  uint4 v = z4();
+  if (c < M) v = h_top(X, M, b, c, tt);
+  if (c >= 1 && c - 1 < M) v = x4(v, h_top(X, M, b + 16, c - 1, tt));
+  return v;
+}
+
+__global__ void qc_output(const uint4* __restrict__ X, int64_t M, int m, int64_t XS, const QcIo* __restrict__ io,
+                          QcIo io1) {
+  const QcIo q = io ? io[blockIdx.y] : io1;
+  const uint4* x0 = q.x;
+  const int64_t n = q.n;
+  uint4* y = q.y;
+  X += (int64_t)blockIdx.y * XS;
+  TopTerms tt;
+  tt.ne = lower_terms(m - 1, tt.e);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    uint4 a = hcontrib(X, M, i), b = hcontrib(X, M, i + n), x = x0[i];
-    y[i] = make_uint4(x.x ^ a.x ^ b.x, x.y ^ a.y ^ b.y, x.z ^ a.z ^ b.z, x.w ^ a.w ^ b.w);
+    y[i] = x4(x0[i], x4(hcontrib(X, M, i, tt), hcontrib(X, M, i + n, tt)));
   }
 }
 
@@ -252,32 +515,229 @@ const Beta& beta() {
   return g_beta;
 }
 
-// forward transform of the packed lower half (planes 0..15) into X (32 planes)
-hipError_t forward(uint32_t* X, int m, hipStream_t st) {
-  const int64_t M = (int64_t)1 << m;
-  const int64_t Mh = M >> 1;
-  for (int k = m - 2; k >= 0; k--) {
-    int64_t nchunk = Mh >> (k + 1);
-    hipLaunchKernelGGL(qc_hi_fix, dim3(grid_for(16 * nchunk * ((int64_t)1 << (k > 0 ? k - 1 : 0)) * 4)),
-                       dim3(256), 0, st, X, M, 16, Mh, k);
-    hipLaunchKernelGGL(qc_lo_update, dim3(grid_for(16 * (Mh >> 1) * 4)), dim3(256), 0, st, X, M, 16, Mh, k);
+int g_sm_configured = 0;
+
+hipError_t configure_lds() {
+  if (g_sm_configured) return hipSuccess;
+  {
+    hipError_t e0 = hipMemcpyToSymbol(HIP_SYMBOL(c_beta), beta().v, sizeof(uint32_t) * 32);
+    if (e0 != hipSuccess) return e0;
   }
-  hipLaunchKernelGGL(qc_dup, dim3(grid_for(32 * Mh)), dim3(256), 0, st, (uint4*)X, M);
-  for (int k = m - 2; k >= 0; k--)
-    hipLaunchKernelGGL(qc_fft_layer<false>, dim3(grid_for(M * 2)), dim3(256), 0, st, X, M, k, beta());
+  hipError_t e = hipFuncSetAttribute((const void*)qc_fft_pass, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     FFT_TC * FFT_STRIDE * 4);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)qc_conv_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          ((int64_t)1 << ROW_MP_MAX) / 8);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)qc_conv_lds<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (1 << CONV_B) * 16);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)qc_conv_lds<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (1 << CONV_B) * 16);
+  if (e != hipSuccess) return e;
+  g_sm_configured = 1;
+  return hipSuccess;
+}
+
+struct Bufs {
+  uint32_t* X;   // B x 32 planes x M x 16 B (element layout), instance stride XS blocks
+  uint4* Y;      // CONV_G planes x M x 16 B (ping-pong of the element-level path, B == 1)
+  uint32_t* Bw;  // B x 32 planes x 128 rows x M bits (row layout)
+  int B;
+  int64_t XS;    // blocks per instance in X (and 16-byte units per instance in Bw)
+};
+
+Bufs bufs(void* scratch, int64_t M, int B) {
+  Bufs b;
+  b.B = B;
+  b.XS = 32 * M;
+  b.X = (uint32_t*)scratch;
+  b.Bw = (uint32_t*)((char*)scratch + (size_t)B * 32 * M * 16);
+  b.Y = (uint4*)((char*)scratch + (size_t)B * 64 * M * 16);
+  return b;
+}
+
+int conv_b(int mpoly) { return mpoly < CONV_B ? mpoly : CONV_B; }
+
+unsigned capg(int64_t v, int64_t cap) { return (unsigned)(v < 1 ? 1 : (v > cap ? cap : v)); }
+
+// forward conversion of planes [0, P), element-level path (single instance)
+hipError_t conv_forward_elem(Bufs b, int64_t M, int P, int mp, hipStream_t st) {
+  const int64_t Mp = (int64_t)1 << mp;
+  const int B = conv_b(mp);
+  for (int p0 = 0; p0 < P; p0 += CONV_G) {
+    const int G = P - p0 < CONV_G ? P - p0 : CONV_G;
+    uint4* x = (uint4*)b.X + (int64_t)p0 * M;
+    const uint4* cur = x;
+    for (int k = mp - 1; k >= B; k--) {
+      uint4* nxt = (cur == x) ? b.Y : x;
+      hipLaunchKernelGGL(qc_conv_level<false>, dim3(grid_for((int64_t)G * Mp, 1 << 14)), dim3(256), 0, st, cur, nxt,
+                         M, G, Mp, k);
+      cur = nxt;
+    }
+    if (B > 0)
+      hipLaunchKernelGGL(qc_conv_lds<false>, dim3(capg((int64_t)G * (Mp >> B), 8192)), dim3(512),
+                         (size_t)(1 << B) * 16, st, cur, x, M, G, Mp, B);
+  }
   return hipGetLastError();
 }
 
-hipError_t inverse(uint32_t* X, int m, hipStream_t st) {
-  const int64_t M = (int64_t)1 << m;
-  for (int k = 0; k <= m - 1; k++)
-    hipLaunchKernelGGL(qc_fft_layer<true>, dim3(grid_for(M * 2)), dim3(256), 0, st, X, M, k, beta());
-  for (int k = 0; k <= m - 1; k++) {
-    int64_t nchunk = M >> (k + 1);
-    hipLaunchKernelGGL(qc_lo_update, dim3(grid_for(32 * (M >> 1) * 4)), dim3(256), 0, st, X, M, 32, M, k);
-    hipLaunchKernelGGL(qc_hi_fix, dim3(grid_for(32 * nchunk * ((int64_t)1 << (k > 0 ? k - 1 : 0)) * 4)),
-                       dim3(256), 0, st, X, M, 32, M, k);
+hipError_t conv_inverse_elem(Bufs b, int64_t M, int P, int mtop, hipStream_t st) {
+  const int64_t Mp = M;
+  const int B = conv_b(mtop);
+  for (int p0 = 0; p0 < P; p0 += CONV_G) {
+    const int G = P - p0 < CONV_G ? P - p0 : CONV_G;
+    uint4* x = (uint4*)b.X + (int64_t)p0 * M;
+    const int nglob = mtop - B;
+    uint4* first_dst = (nglob & 1) ? b.Y : x;
+    if (B > 0)
+      hipLaunchKernelGGL(qc_conv_lds<true>, dim3(capg((int64_t)G * (Mp >> B), 8192)), dim3(512),
+                         (size_t)(1 << B) * 16, st, x, first_dst, M, G, Mp, B);
+    const uint4* cur = (B > 0) ? first_dst : x;
+    for (int k = B; k < mtop; k++) {
+      uint4* nxt = (cur == x) ? b.Y : x;
+      hipLaunchKernelGGL(qc_conv_level<true>, dim3(grid_for((int64_t)G * Mp, 1 << 14)), dim3(256), 0, st, cur, nxt,
+                         M, G, Mp, k);
+      cur = nxt;
+    }
   }
+  return hipGetLastError();
+}
+
+bool row_path(int mp) { return mp >= ROW_MP_MIN && mp <= ROW_MP_MAX; }
+
+// forward conversion of planes [0, P) of the lower-half polynomial (2^mp
+// elements) of every instance, through the row layout
+hipError_t conv_forward(Bufs b, int64_t M, int P, int mp, hipStream_t st) {
+  if (!row_path(mp)) return conv_forward_elem(b, M, P, mp, st);
+  const int64_t Mp = (int64_t)1 << mp;
+  const int64_t row_words = Mp / 32;
+  const int64_t inst_words = b.XS * 4;  // Bw instance stride (same bytes as X)
+  hipError_t e = cuot_launch_transpose_planes2(b.X, Mp, b.Bw, false, b.B * P, P, b.XS, M, inst_words,
+                                               128 * row_words, st);
+  if (e != hipSuccess) return e;
+  const int64_t tpi = (int64_t)P * 128;
+  hipLaunchKernelGGL(qc_conv_rows, dim3(capg(tpi * b.B, 4096)), dim3(1024), (size_t)row_words * 4, st, b.Bw,
+                     row_words, tpi * b.B, tpi, inst_words, mp, 0);
+  return cuot_launch_transpose_planes2(b.Bw, Mp, b.X, true, b.B * P, P, inst_words, 128 * row_words, b.XS, M, st);
+}
+
+// inverse conversion levels 0..mtop-1 of planes [0, P) of the full polynomial
+// (2^(mtop+1) elements): each half string is converted on its own
+hipError_t conv_inverse(Bufs b, int64_t M, int P, int mtop, hipStream_t st) {
+  if (!row_path(mtop)) return conv_inverse_elem(b, M, P, mtop, st);
+  const int64_t row_words = M / 32;
+  const int64_t inst_words = b.XS * 4;
+  hipError_t e = cuot_launch_transpose_planes2(b.X, M, b.Bw, false, b.B * P, P, b.XS, M, inst_words,
+                                               128 * row_words, st);
+  if (e != hipSuccess) return e;
+  const int64_t tile_words = row_words / 2;
+  const int64_t tpi = (int64_t)P * 128 * 2;
+  hipLaunchKernelGGL(qc_conv_rows, dim3(capg(tpi * b.B, 8192)), dim3(1024), (size_t)tile_words * 4, st, b.Bw,
+                     tile_words, tpi * b.B, tpi, inst_words, mtop, 1);
+  return cuot_launch_transpose_planes2(b.Bw, M, b.X, true, b.B * P, P, inst_words, 128 * row_words, b.XS, M, st);
+}
+
+void pass_grid(int64_t ntiles, unsigned& g) { g = (unsigned)(ntiles < 4096 ? ntiles : 4096); }
+
+FftPassArgs pass_args(const Bufs& b, int64_t M, int lo, int R, int C, int mode, int khi, int first,
+                      const uint32_t* ahat1, const uint32_t* const* ahats) {
+  FftPassArgs a;
+  a.X = b.X;
+  a.M = M;
+  a.lo = lo;
+  a.R = R;
+  a.C = C;
+  a.lgC = __builtin_ctz(C);
+  a.mode = mode;
+  a.khi = khi;
+  a.first = first;
+  a.ahat = ahat1;
+  a.ahats = ahats;
+  a.XS_words = b.XS * 4;
+  return a;
+}
+
+void launch_pass(const FftPassArgs& a, int B, hipStream_t st) {
+  unsigned g;
+  pass_grid(a.M / ((1 << a.R) * a.C), g);
+  hipLaunchKernelGGL(qc_fft_pass, dim3(g, (unsigned)B), dim3(FFT_THREADS), (size_t)(1 << a.R) * a.C * FFT_STRIDE * 4,
+                     st, a);
+}
+
+// forward FFT layers m-1..0 (top layer from the packed lower half); with A^,
+// the lowest pass also multiplies by A^ and runs the inverse layers of its
+// range.  Returns the highest layer of the lowest pass in *low_khi.
+hipError_t fft_forward(const Bufs& b, int m, const uint32_t* ahat1, const uint32_t* const* ahats, bool fused,
+                       int* low_khi, hipStream_t st) {
+  const int64_t M = (int64_t)1 << m;
+  const int R0 = m < 8 ? m : 8;
+  int khi = m - 1;
+  bool first = true;
+  while (khi >= R0) {
+    int R = khi - R0 + 1 < 5 ? khi - R0 + 1 : 5;
+    int lo = khi - R + 1;
+    int C = 8;
+    while (C > 1 && (((int64_t)1 << lo) % C)) C >>= 1;
+    while (((1 << R) * C) > FFT_TC) C >>= 1;
+    launch_pass(pass_args(b, M, lo, R, C, PASS_FWD, khi, first ? 1 : 0, nullptr, nullptr), b.B, st);
+    first = false;
+    khi = lo - 1;
+  }
+  launch_pass(pass_args(b, M, 0, R0, 1, fused ? PASS_FUSED : PASS_FWD, R0 - 1, first ? 1 : 0, ahat1, ahats), b.B, st);
+  *low_khi = R0 - 1;
+  return hipGetLastError();
+}
+
+hipError_t fft_inverse_upper(const Bufs& b, int m, int from, hipStream_t st) {
+  const int64_t M = (int64_t)1 << m;
+  int klo = from;
+  while (klo <= m - 1) {
+    int R = m - klo < 5 ? m - klo : 5;
+    int C = 8;
+    while (C > 1 && (((int64_t)1 << klo) % C)) C >>= 1;
+    while (((1 << R) * C) > FFT_TC) C >>= 1;
+    launch_pass(pass_args(b, M, klo, R, C, PASS_INV, klo + R - 1, 0, nullptr, nullptr), b.B, st);
+    klo += R;
+  }
+  return hipGetLastError();
+}
+
+hipError_t encode_batch(const QcIo* io, QcIo io1, const uint32_t* ahat1, const uint32_t* const* ahats, int B, int m,
+                        void* scratch, hipStream_t st) {
+  hipError_t e = configure_lds();
+  if (e != hipSuccess) return e;
+  if (B > 1 && !row_path(m - 1)) return hipErrorInvalidValue;
+  const int64_t M = (int64_t)1 << m;
+  Bufs b = bufs(scratch, M, B);
+  hipLaunchKernelGGL(qc_pack_x, dim3(grid_for(16 * (M >> 1)), (unsigned)B), dim3(256), 0, st, io, io1, (uint4*)b.X, M,
+                     b.XS);
+  if ((e = conv_forward(b, M, 16, m - 1, st))) return e;
+  int khi;
+  if ((e = fft_forward(b, m, ahat1, ahats, true, &khi, st))) return e;
+  if ((e = fft_inverse_upper(b, m, khi + 1, st))) return e;
+  if ((e = conv_inverse(b, M, 31, m - 1, st))) return e;
+  const int64_t nmax = io ? (M / 2) * 16 : io1.n;  // grid covers the largest n of the class
+  hipLaunchKernelGGL(qc_output, dim3(grid_for(nmax), (unsigned)B), dim3(256), 0, st, (const uint4*)b.X, M, m, b.XS,
+                     io, io1);
+  return hipGetLastError();
+}
+
+hipError_t prepare_batch(const QcPrep* pr, QcPrep pr1, int B, int m, hipStream_t st) {
+  hipError_t e = configure_lds();
+  if (e != hipSuccess) return e;
+  const int64_t M = (int64_t)1 << m;
+  const int64_t Mh = M >> 1;
+  const dim3 blk(256);
+  hipLaunchKernelGGL(qcs_pack, dim3(grid_for(M), (unsigned)B), blk, 0, st, pr, pr1, M);
+  for (int k = m - 2; k >= 0; k--) {
+    hipLaunchKernelGGL(qcs_hi_fix, dim3(grid_for(Mh >> 1, 4096), (unsigned)B), blk, 0, st, pr, pr1, Mh, k);
+    hipLaunchKernelGGL(qcs_lo_update, dim3(grid_for(Mh >> 1, 4096), (unsigned)B), blk, 0, st, pr, pr1, Mh, k);
+  }
+  // top layer (constant 0, empty upper half): duplicate the lower half
+  hipLaunchKernelGGL(qcs_fft_layer, dim3(grid_for(Mh, 4096), (unsigned)B), blk, 0, st, pr, pr1, M, m - 1);
+  for (int k = m - 2; k >= 0; k--)
+    hipLaunchKernelGGL(qcs_fft_layer, dim3(grid_for(Mh, 4096), (unsigned)B), blk, 0, st, pr, pr1, M, k);
   return hipGetLastError();
 }
 
@@ -290,34 +750,43 @@ int cuot_qc_log_len(int64_t n) {
   return m;
 }
 
-size_t cuot_qc_scratch_bytes_for(int64_t n) {
-  const int m = cuot_qc_log_len(n);
-  return (size_t)32 * ((size_t)1 << m) * 16;
+size_t cuot_qc_scratch_bytes_for(int64_t n) { return cuot_qc_batch_scratch_bytes(cuot_qc_log_len(n), 1); }
+
+bool cuot_qc_batchable(int m) { return row_path(m - 1); }
+
+hipError_t cuot_qc_configure() { return configure_lds(); }
+
+size_t cuot_qc_batch_scratch_bytes(int m, int B) {
+  return ((size_t)B * 64 + CONV_G) * ((size_t)1 << m) * 16;
 }
 
 hipError_t cuot_launch_qc_prepare(const uint8_t* a_bits, int64_t n, uint32_t* ahat, void* scratch, hipStream_t st) {
-  const int m = cuot_qc_log_len(n);
-  const int64_t M = (int64_t)1 << m;
-  uint32_t* X = (uint32_t*)scratch;
-  hipLaunchKernelGGL(qc_pack_a, dim3(grid_for(16 * (M >> 1))), dim3(256), 0, st, a_bits, n, (uint4*)X, M);
-  hipError_t e = forward(X, m, st);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(qc_extract_row0, dim3(grid_for(M)), dim3(256), 0, st, X, M, ahat);
-  return hipGetLastError();
+  (void)scratch;
+  QcPrep p1;
+  p1.a_bits = a_bits;
+  p1.ahat = ahat;
+  p1.n = n;
+  return prepare_batch(nullptr, p1, 1, cuot_qc_log_len(n), st);
+}
+
+hipError_t cuot_launch_qc_prepare_batch(const QcPrep* prep_d, int B, int m, hipStream_t st) {
+  QcPrep p1;
+  memset(&p1, 0, sizeof(p1));
+  return prepare_batch(prep_d, p1, B, m, st);
 }
 
 hipError_t cuot_launch_qc_encode(const uint32_t* ahat, int64_t n, const void* x, void* y, void* scratch,
                                  hipStream_t st) {
-  const int m = cuot_qc_log_len(n);
-  const int64_t M = (int64_t)1 << m;
-  uint32_t* X = (uint32_t*)scratch;
-  const uint4* x0 = (const uint4*)x;
-  hipLaunchKernelGGL(qc_pack_x, dim3(grid_for(16 * (M >> 1))), dim3(256), 0, st, x0 + n, n, (uint4*)X, M);
-  hipError_t e = forward(X, m, st);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(qc_pointwise, dim3(grid_for(M * 4)), dim3(256), 0, st, X, M, ahat);
-  e = inverse(X, m, st);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(qc_output, dim3(grid_for(n)), dim3(256), 0, st, (const uint4*)X, M, x0, n, (uint4*)y);
-  return hipGetLastError();
+  QcIo io1;
+  io1.x = (const uint4*)x;
+  io1.y = (uint4*)y;
+  io1.n = n;
+  return encode_batch(nullptr, io1, ahat, nullptr, 1, cuot_qc_log_len(n), scratch, st);
+}
+
+hipError_t cuot_launch_qc_encode_batch(const QcIo* io_d, const uint32_t* const* ahats_d, int B, int m, void* scratch,
+                                       hipStream_t st) {
+  QcIo io1;
+  memset(&io1, 0, sizeof(io1));
+  return encode_batch(io_d, io1, nullptr, ahats_d, B, m, scratch, st);
 }

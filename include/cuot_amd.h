@@ -158,7 +158,22 @@ typedef struct {
   cuot_block seed;
 } cuot_sot_config;
 
+enum {
+  CUOT_SOT_STAGE_SETUP = 1,      /* seed -> Delta, roots, alpha, a, base-OT messages */
+  CUOT_SOT_STAGE_QC_PREPARE = 2, /* A^ = transform of a */
+  CUOT_SOT_STAGE_GGM_SENDER = 4, /* sender GGM + masked rows + receiver's view */
+  CUOT_SOT_STAGE_GGM_RECVER = 8, /* receiver GGM */
+  CUOT_SOT_STAGE_QC_SENDER = 16,
+  CUOT_SOT_STAGE_QC_RECVER = 32,
+  CUOT_SOT_STAGE_SPARSE = 64,
+  CUOT_SOT_STAGE_ALL = 127
+};
+
 size_t cuot_sot_workspace_bytes(const cuot_sot_config *cfg);
+/* Runs the selected stages in order (for per-stage timing; later stages read
+ * what earlier ones left in the workspace). */
+int cuot_sot_run_stages(cuot_ctx *ctx, const cuot_sot_config *cfg, void *ws_d, size_t ws_bytes, cuot_block *v_d,
+                        cuot_block *w_d, uint8_t *b_d, cuot_block *delta_d, uint32_t stages, void *stream);
 int cuot_sot_run(cuot_ctx *ctx, const cuot_sot_config *cfg, void *ws_d, size_t ws_bytes, cuot_block *v_d,
                  cuot_block *w_d, uint8_t *b_d, cuot_block *delta_d, void *stream);
 
@@ -175,9 +190,128 @@ typedef struct {
   cuot_block seed;
 } cuot_ferret_config;
 
+enum {
+  CUOT_FERRET_STAGE_SETUP = 1,
+  CUOT_FERRET_STAGE_GGM_SENDER = 2,
+  CUOT_FERRET_STAGE_GGM_RECVER = 4,
+  CUOT_FERRET_STAGE_LPN = 8,
+  CUOT_FERRET_STAGE_ALL = 15
+};
+
 size_t cuot_ferret_workspace_bytes(const cuot_ferret_config *cfg);
+int cuot_ferret_run_stages(cuot_ctx *ctx, const cuot_ferret_config *cfg, void *ws_d, size_t ws_bytes,
+                           cuot_block *out_s_d, cuot_block *out_r_d, cuot_block *delta_d, uint32_t stages,
+                           void *stream);
 int cuot_ferret_run(cuot_ctx *ctx, const cuot_ferret_config *cfg, void *ws_d, size_t ws_bytes,
                     cuot_block *out_s_d, cuot_block *out_r_d, cuot_block *delta_d, void *stream);
+
+/* ---- drop-in: emp-ot Ferret dev_layer ----------------------------------------
+ * Same argument meaning and blocking, host-in/host-out behaviour as
+ * ferret/emp-ot/emp-ot/ferret/dev_layer.h:9-18, with Mat* replaced by arrays
+ * of device pointers (one per GPU; this build drives ngpu == 1) and emp-ot's
+ * semantics (TwoKeyPRP keys, AES(s)^s children, LSB-cleared leaves, secret
+ * sum) in place of dev_layer.cu's deviations.  Return CUOT_OK or an error. */
+int cuot_setdev(int gpu); /* cuda_setdev, dev_layer.h:9 */
+
+/* cuda_mpcot_sender, dev_layer.h:11-12.  expanded_d[0]: t*2^depth blocks;
+ * on entry its first t blocks hold the tree roots (the reference's level loop
+ * reads its level-0 input from there, dev_layer.cu:55-58), on return the
+ * leaves.  buffer_d/sep_d: caller scratch of t*2^depth blocks
+ * (mpcot_reg.h:55-61).  lSum_h/rSum_h[tree*depth + level], secret_sum_h[tree]
+ * are written on the host. */
+int cuot_mpcot_sender(cuot_ctx *ctx, cuot_block *const *expanded_d, cuot_block *const *buffer_d,
+                      cuot_block *const *sep_d, cuot_block *lSum_h, cuot_block *rSum_h, cuot_block *secret_sum_h,
+                      const int *t, int depth, const cuot_block *delta_h, int ngpu);
+/* cuda_mpcot_recver, dev_layer.h:14-15.  cSum_h[tree*depth+level]: chosen
+ * sums; choices_h[tree*depth+level]: bool bytes (mpcot_reg.h:352-363). */
+int cuot_mpcot_recver(cuot_ctx *ctx, cuot_block *const *expanded_d, cuot_block *const *buffer_d,
+                      cuot_block *const *sep_d, const cuot_block *cSum_h, const cuot_block *secret_sum_h,
+                      const int *t, int depth, const uint8_t *choices_h, int ngpu);
+/* cuda_primal_lpn, dev_layer.h:17-18.  key: emp AES_KEY.rd_key of PRP(seed)
+ * (first 16 bytes = seed); expanded_d[0]: noise in, LPN output out (n[0]
+ * blocks); nn_h: host copy of the output; kk_d[0]: device scratch of k blocks
+ * receiving kk_h; pubMats_d is not used (indices are never materialised). */
+int cuot_primal_lpn(cuot_ctx *ctx, int role, cuot_block *const *pubMats_d, int64_t d, const int64_t *n, int k,
+                    const uint32_t *key, cuot_block *const *expanded_d, cuot_block *nn_h, cuot_block *const *kk_d,
+                    const cuot_block *kk_h, int ngpu);
+
+/* ---- drop-in: SOT roles (silent/roles.h:15-85) -------------------------------
+ * SilentConfig fields; logOT = log2 of the number of GGM leaves (2n), depth =
+ * logOT - log2(nTree).  Base OTs are simulated from `seed` (both roles must be
+ * created with the same seed).  Call order (silent/main.cu:22-72):
+ * sender base_ot, receiver base_ot, sender seed_expand, receiver get_punc_key,
+ * receiver seed_expand, dual_lpn on both.  Outputs stay device-resident. */
+typedef struct {
+  int32_t id;
+  int32_t logOT;
+  uint64_t nTree;
+  uint32_t leftKey[4];
+  uint32_t rightKey[4];
+  int32_t gpuPerParty;
+  int32_t reserved;
+} cuot_silent_config;
+
+typedef struct cuot_sot_role cuot_sot_role;
+
+int cuot_sot_sender_new(cuot_ctx *ctx, const cuot_silent_config *cfg, const cuot_block *seed_h,
+                        cuot_sot_role **out); /* SOTSender::SOTSender, sender.cu:102-136 */
+int cuot_sot_recver_new(cuot_ctx *ctx, const cuot_silent_config *cfg, const cuot_block *seed_h,
+                        cuot_sot_role **out); /* SOTRecver::SOTRecver, recver.cu:220-256 */
+int cuot_sot_role_free(cuot_sot_role *r);
+int cuot_sot_base_ot(cuot_sot_role *r);       /* base_ot, sender.cu:152-164 / recver.cu:273-285 */
+int cuot_sot_seed_expand(cuot_sot_role *r);   /* seed_expand, sender.cu:166-198 / recver.cu:333-372 */
+int cuot_sot_get_punc_key(cuot_sot_role *recv, const cuot_sot_role *send); /* recver.cu:326-331 */
+int cuot_sot_dual_lpn(cuot_sot_role *r);      /* dual_lpn, sender.cu:200-209 / recver.cu:374-383 */
+/* fullVector / puncVector (n blocks), choiceVector (receiver), delta (sender) */
+int cuot_sot_outputs(const cuot_sot_role *r, cuot_block **vec_d, uint8_t **choice_bits_d, cuot_block **delta_d,
+                     int64_t *n);
+
+/* ---- batches of instances ------------------------------------------------------
+ * No counterpart in the reference (which runs one instance per process): many
+ * SOT and Ferret instances of different sizes run as one launch sequence, the
+ * GGM trees, LPN rows and QC transforms of all instances sharing launches.
+ * Every instance's outputs are byte-identical to cuot_sot_run / cuot_ferret_run
+ * of the same instance.  Life cycle: new (plan, host only) -> bind (workspace
+ * and outputs; uploads the launch tables, synchronous) -> run (any number of
+ * times, on a stream; CUOT_BATCH_GRAPH replays a HIP graph captured on the
+ * first such call) -> free. */
+typedef struct {
+  int32_t kind;   /* CUOT_MODE_SOT or CUOT_MODE_FERRET */
+  int32_t h;      /* tree depth */
+  int64_t n;      /* COTs */
+  int64_t t;      /* trees */
+  int64_t k;      /* Ferret bootstrap COTs (0 for SOT) */
+  int32_t ntasks; /* Ferret LPN task partition (emp-ot threads) */
+  int32_t reserved;
+  cuot_block k0, k1; /* SOT GGM keys (ignored for Ferret: TwoKeyPRP keys) */
+  cuot_block seed;
+} cuot_instance;
+
+enum {
+  CUOT_BATCH_STAGE_SETUP = 1,      /* seeds -> Delta, roots, alpha, a, base-OT messages, bootstrap */
+  CUOT_BATCH_STAGE_QC_PREPARE = 2, /* A^ of every SOT instance */
+  CUOT_BATCH_STAGE_GGM_SENDER = 4, /* sender GGM of all instances + messages */
+  CUOT_BATCH_STAGE_GGM_RECVER = 8, /* receiver GGM of all instances */
+  CUOT_BATCH_STAGE_QC = 16,        /* QC encode of both SOT parties */
+  CUOT_BATCH_STAGE_SPARSE = 32,    /* SOT choice bits */
+  CUOT_BATCH_STAGE_LPN = 64,       /* Ferret LPN of both parties */
+  CUOT_BATCH_STAGE_ALL = 127,
+  CUOT_BATCH_GRAPH = 1 << 16       /* run flag: replay a captured HIP graph */
+};
+
+typedef struct cuot_batch cuot_batch;
+/* knobs: qc_chunk_bytes caps the shared QC scratch of one encode launch
+ * sequence (0 = default). */
+int cuot_batch_new(cuot_ctx *ctx, const cuot_instance *inst, int count, size_t qc_chunk_bytes, cuot_batch **out);
+size_t cuot_batch_workspace_bytes(const cuot_batch *b);
+/* out0[i]: SOT v / Ferret sender (n_i blocks); out1[i]: SOT w / Ferret
+ * receiver; bits[i]: SOT choice bits ((n_i+31)/32 words; NULL for Ferret);
+ * delta_d: count blocks (may be NULL).  Pointer arrays are host arrays of
+ * device pointers. */
+int cuot_batch_bind(cuot_batch *b, void *ws_d, size_t ws_bytes, cuot_block *const *out0, cuot_block *const *out1,
+                    uint8_t *const *bits, cuot_block *delta_d);
+int cuot_batch_run(cuot_batch *b, uint32_t stages, void *stream);
+int cuot_batch_free(cuot_batch *b);
 
 #ifdef __cplusplus
 }

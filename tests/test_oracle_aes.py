@@ -89,3 +89,23 @@ def test_prg_is_ctr_of_counter_blocks():
     ctr[:, 0] = np.arange(100, 109, dtype=np.uint64)
     ctr[:, 1] = 7
     assert np.array_equal(out, po.aes_ecb(seed, ctr))
+
+
+def test_matches_reference_cryptotools_aes():
+    """The reference's vendored cryptoTools AES (compiled into oracle/_ref from
+    the reference's own sources) agrees with the oracle at random keys."""
+    import ctypes
+    import os
+
+    path = os.path.join(os.path.dirname(po.__file__), "_ref", "libref_aes.so")
+    if not os.path.exists(path):
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    L = ctypes.CDLL(path)
+    L.ref_aes_ecb.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    r = rng(11)
+    for _ in range(5):
+        key = rand_blocks(r, 1)
+        x = rand_blocks(r, 100)
+        out = np.zeros_like(x)
+        L.ref_aes_ecb(key.ctypes.data, x.ctypes.data, out.ctypes.data, 100)
+        assert np.array_equal(out, po.aes_ecb(key, x))

@@ -39,6 +39,11 @@ def parse():
     ap.add_argument("--no-ferret", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--sot-n-log", type=int, default=24)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched (64-instance) leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     return ap.parse_args()
 
 
@@ -68,6 +73,120 @@ def cpu_baseline(threads: int):
         "sample": f"oracle SOT n=2^20 t=256 h=13 both roles + QC, median of 3 runs after 1 warm-up "
                   f"({med:.2f} s/run)",
     }
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = [p.strip() for p in line.strip().strip("|").split("|")]
+        if len(parts) >= 4 and parts[0] == kernel:
+            try:
+                return float(parts[3])
+            except ValueError:
+                return None
+    return None
+
+
+def batch_leg(torch, cuot_amd, ctx, steps):
+    """64 instances (log-uniform 2^16..2^22, SOT/Ferret alternating): plain
+    one-instance-at-a-time runs vs the batched engine replayed as a graph."""
+    from cuot_amd.workloads import SOT, batch_mix
+
+    mix = batch_mix(64)
+    insts = [cuot_amd.make_instance_struct(wi.kind, wi.n, wi.t, wi.h, wi.k, wi.seed, k0=SOT_K0, k1=SOT_K1)
+             for wi in mix]
+    total = sum(wi.n for wi in mix)
+    dev = "cuda"
+    # plain: per-instance workspaces and outputs, one stream, instances in order
+    plain = []
+    for wi, ins in zip(mix, insts):
+        if wi.kind == SOT:
+            cfg = cuot_amd.SotConfig(n=wi.n, t=wi.t, h=wi.h, reserved=0, k0=ins.k0, k1=ins.k1, seed=ins.seed)
+            ws = torch.empty(ctx.sot_workspace_bytes(cfg), dtype=torch.uint8, device=dev)
+            b = torch.empty(((wi.n + 31) // 32) * 4, dtype=torch.uint8, device=dev)
+        else:
+            cfg = cuot_amd.FerretConfig(n=wi.n, k=wi.k, t=wi.t, h=wi.h, ntasks=ins.ntasks, seed=ins.seed)
+            ws = torch.empty(ctx.ferret_workspace_bytes(cfg), dtype=torch.uint8, device=dev)
+            b = None
+        o0 = torch.empty((wi.n, 2), dtype=torch.int64, device=dev)
+        o1 = torch.empty((wi.n, 2), dtype=torch.int64, device=dev)
+        d = torch.empty((1, 2), dtype=torch.int64, device=dev)
+        plain.append((wi, cfg, ws, o0, o1, b, d))
+    stream = torch.cuda.Stream()
+
+    def run_plain():
+        for wi, cfg, ws, o0, o1, b, d in plain:
+            if wi.kind == SOT:
+                ctx.sot_run(cfg, ws, o0, o1, b, d, stream)
+            else:
+                ctx.ferret_run(cfg, ws, o0, o1, d, stream)
+
+    bt = cuot_amd.Batch(ctx, insts)
+
+    def run_batch():
+        bt.run(graph=True, stream=stream)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    plain_ms = timed(run_plain)
+    batch_ms = timed(run_batch)
+    exact = True
+    for i, (wi, cfg, ws, o0, o1, b, d) in enumerate(plain):
+        exact &= bool(torch.equal(o0, bt.out0[i])) and bool(torch.equal(o1, bt.out1[i]))
+        exact &= bool(torch.equal(d[0], bt.delta[i]))
+        if wi.kind == SOT:
+            exact &= bool(torch.equal(b[: (wi.n + 7) // 8], bt.bits[i].view(torch.uint8)[: (wi.n + 7) // 8]))
+    out = {
+        "config": {"workload": "batched mix", "instances": len(mix), "sizes": "n = floor(2^(16 + 6u))",
+                   "kinds": "SOT / Ferret alternating", "total_cots": total},
+        "plain": {"value": round(total / (plain_ms / 1e3), 1), "unit": "COTs/s", "ms": round(plain_ms, 3)},
+        "batched": {"value": round(total / (batch_ms / 1e3), 1), "unit": "COTs/s", "ms": round(batch_ms, 3),
+                    "form": "one launch sequence per stage over all instances, HIP graph replay"},
+        "speedup": round(plain_ms / batch_ms, 3),
+        "byte_exact_vs_plain": exact,
+    }
+    bt.close()
+    del plain
+    return out
+
+
+def serve_leg(torch, cuot_amd, ctx, count, rate, budget_gb):
+    """Poisson arrivals, n = floor(2^(14 + 10u)), 50/50 SOT/Ferret: continuous
+    batching under an HBM budget vs the FIFO plain loop, real-time replay."""
+    import numpy as np
+
+    from cuot_amd.workloads import serving_arrivals
+
+    arr = serving_arrivals(rate, count)
+    insts = [cuot_amd.make_instance_struct(wi.kind, wi.n, wi.t, wi.h, wi.k, wi.seed, k0=SOT_K0, k1=SOT_K1)
+             for _, wi in arr]
+    times = [a for a, _ in arr]
+    total = sum(wi.n for _, wi in arr)
+    budget = int(budget_gb * (1 << 30))
+    res = {"config": {"workload": "serving", "requests": count, "rate_hz": rate,
+                      "sizes": "n = floor(2^(14 + 10u))", "kinds": "SOT / Ferret 50/50", "hbm_budget_bytes": budget}}
+    for name, plain in (("served", False), ("fifo_plain", True)):
+        torch.cuda.synchronize()
+        lat, wall, nb = cuot_amd.serve_replay(ctx, insts, times, hbm_budget=budget, plain=plain,
+                                              max_batch=64, streams=2)
+        res[name] = {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                     "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                     "mean_ms": round(float(lat.mean()) * 1e3, 3),
+                     "throughput_cots_s": round(total / wall, 1), "wall_s": round(wall, 3), "batches": nb}
+    return res
 
 
 def relation_ok(torch, v, w, b, d, n):
@@ -115,20 +234,40 @@ def main():
     torch.cuda.synchronize()
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)] for _ in range(args.steps)]
     torch.cuda.synchronize()
+    lib = ctx.lib
+    import ctypes as C
+
+    cuot_amd.check(lib.cuot_timing_begin(ctx.handle), "cuot_timing_begin")
     t0 = time.perf_counter()
     for s in range(args.steps):
         sot_step(ev[s])
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    recs = (cuot_amd._lib.LaunchTime * 4096)()
+    nrec = C.c_int()
+    cuot_amd.check(lib.cuot_timing_end(ctx.handle, recs, 4096, C.byref(nrec)), "cuot_timing_end")
     ms_per_step = (t1 - t0) * 1e3 / args.steps
+    kernels = {}
+    for r in recs[: nrec.value]:
+        k = kernels.setdefault(r.kernel.decode(), {"launches": 0, "ms": 0.0, "bytes": 0.0})
+        k["launches"] += 1
+        k["ms"] += r.ms
+        k["bytes"] += r.bytes
+    per_kernel = {name: {"launches_per_step": k["launches"] / args.steps,
+                         "ms_per_step": round(k["ms"] / args.steps, 4),
+                         "avg_launch_ms": round(k["ms"] / k["launches"], 4),
+                         "algorithmic_GBps": round(k["bytes"] / (k["ms"] / 1e3) / 1e9, 1) if k["ms"] > 0 else None}
+                  for name, k in kernels.items()}
+    dom_name, dom = max(kernels.items(), key=lambda kv: kv[1]["ms"])
     stage_ms = {}
     for i, (name, _) in enumerate(stages):
         stage_ms[name] = round(sum(ev[s][i].elapsed_time(ev[s][i + 1]) for s in range(args.steps)) / args.steps, 4)
     ok = relation_ok(torch, v, w, b, d, n)
 
-    # dominant HBM-resident data movement kernel: the QC encode (per party)
     qc_ms = (stage_ms["qc_sender"] + stage_ms["qc_recver"]) / 2
-    qc_bytes = 48 * n  # read 2n blocks, write n blocks (compulsory)
+    qc_bytes = 48 * n  # compulsory per encode: read 2n blocks, write n blocks
+    dom_achieved = dom["bytes"] / (dom["ms"] / 1e3) / 1e9
+    traffic = measured_traffic(dom_name)
     ggm_ms = (stage_ms["ggm_sender"] + stage_ms["ggm_recver"]) / 2
     aes_per_party = 2 * t * ((1 << h) - 1)
 
@@ -167,6 +306,17 @@ def main():
             "relation_ok": fok,
         }
 
+    # release the headline buffers before the other legs
+    del ws, v, w, b
+    torch.cuda.empty_cache()
+    batched = None
+    if not args.no_batch:
+        batched = batch_leg(torch, cuot_amd, ctx, max(1, args.steps))
+        torch.cuda.empty_cache()
+    serving = None
+    if not args.no_serve:
+        serving = serve_leg(torch, cuot_amd, ctx, args.serve_requests, args.serve_rate, args.serve_budget_gb)
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -192,16 +342,23 @@ def main():
         "stage_ms": stage_ms,
         "roofline": {
             "bound": "hbm",
-            "kernel": "qc_encode (per party, all launches of one encode)",
-            "achieved": round(qc_bytes / (qc_ms / 1e3) / 1e9, 2),
+            "kernel": dom_name,
+            "achieved": round(dom_achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(qc_bytes / (qc_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "frac": round(dom_achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "avg_launch_ms": round(dom["ms"] / dom["launches"], 4),
+            "algorithmic_bytes_per_launch": round(dom["bytes"] / dom["launches"]),
         },
+        "kernels": per_kernel,
+        "qc_encode": {"ms_per_party": round(qc_ms, 4), "compulsory_bytes": qc_bytes,
+                      "compulsory_GBps": round(qc_bytes / (qc_ms / 1e3) / 1e9, 1)},
         "ggm": {"aes_per_launch": aes_per_party, "ms_per_launch": round(ggm_ms, 4),
                 "aes_per_s": round(aes_per_party / (ggm_ms / 1e3), 1)},
         "ferret": ferret,
+        "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

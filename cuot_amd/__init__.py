@@ -13,12 +13,13 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import Block, CuotError, FerretConfig, GgmParams, Instance, LpnParams, SotConfig, check, load
+from ._lib import (Block, CuotError, FerretConfig, GgmParams, Instance, LpnParams, ServerConfig, ServerResult,
+                   SotConfig, check, load)
 
 MODE_SOT = 0
 MODE_FERRET = 1
 
-__all__ = ["Context", "Batch", "Block", "GgmParams", "LpnParams", "SotConfig", "FerretConfig", "Instance",
+__all__ = ["Context", "Batch", "Server", "ServerConfig", "serve_replay", "Block", "GgmParams", "LpnParams", "SotConfig", "FerretConfig", "Instance",
            "CuotError", "MODE_SOT", "MODE_FERRET", "BATCH_GRAPH", "to_block", "load"]
 
 
@@ -205,6 +206,68 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """The serving loop (cuot_server_*): submit requests, poll results, release."""
+
+    def __init__(self, ctx: Context, hbm_budget: int, max_batch: int = 0, streams: int = 0, qc_chunk_bytes: int = 0,
+                 use_graph: bool = False, plain: bool = False):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.cfg = ServerConfig(hbm_budget=hbm_budget, max_batch=max_batch, streams=streams,
+                                qc_chunk_bytes=qc_chunk_bytes, use_graph=int(use_graph), plain=int(plain))
+        h = C.c_void_p()
+        check(self.lib.cuot_server_new(ctx.handle, C.byref(self.cfg), C.byref(h)), "cuot_server_new")
+        self.handle = h
+
+    def submit(self, inst: Instance) -> int:
+        t = C.c_int64()
+        check(self.lib.cuot_server_submit(self.handle, C.byref(inst), C.byref(t)), "cuot_server_submit")
+        return t.value
+
+    def poll(self, ticket: int, wait: bool = False) -> ServerResult:
+        r = ServerResult()
+        check(self.lib.cuot_server_poll(self.handle, ticket, int(wait), C.byref(r)), "cuot_server_poll")
+        return r
+
+    def release(self, ticket: int):
+        check(self.lib.cuot_server_release(self.handle, ticket), "cuot_server_release")
+
+    def stats(self):
+        b = C.c_int64()
+        u = C.c_size_t()
+        check(self.lib.cuot_server_stats(self.handle, C.byref(b), C.byref(u)), "cuot_server_stats")
+        return b.value, u.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.cuot_server_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def serve_replay(ctx: Context, instances, arrivals, hbm_budget: int, plain: bool = False, max_batch: int = 0,
+                 streams: int = 0, qc_chunk_bytes: int = 0, use_graph: bool = False):
+    """Real-time replay of (arrival seconds, Instance) through a fresh server.
+
+    Returns (latencies in s as numpy array, wall seconds, batches launched)."""
+    n = len(instances)
+    cfg = ServerConfig(hbm_budget=hbm_budget, max_batch=max_batch, streams=streams, qc_chunk_bytes=qc_chunk_bytes,
+                       use_graph=int(use_graph), plain=int(plain))
+    arr = (Instance * n)(*instances)
+    t = (C.c_double * n)(*[float(a) for a in arrivals])
+    lat = (C.c_double * n)()
+    wall = C.c_double()
+    nb = C.c_int64()
+    check(ctx.lib.cuot_serve_replay(ctx.handle, C.byref(cfg), arr, t, n, lat, C.byref(wall), C.byref(nb)),
+          "cuot_serve_replay")
+    return np.array(lat[:], dtype=np.float64), wall.value, nb.value
 
 
 def make_instance_struct(kind: int, n: int, t: int, h: int, k: int, seed, ntasks: int = 1, k0=(3242342, 0),

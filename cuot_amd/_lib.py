@@ -50,6 +50,24 @@ class Instance(C.Structure):
                 ("k0", Block), ("k1", Block), ("seed", Block)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_batch", I32), ("streams", I32), ("qc_chunk_bytes", C.c_size_t),
+                ("use_graph", I32), ("plain", I32)]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("done", I32), ("status", I32), ("t_submit", C.c_double), ("t_start", C.c_double),
+                ("t_done", C.c_double), ("out0", P), ("out1", P), ("bits", P), ("delta", P)]
+
+
+class KnobValues(C.Structure):
+    _fields_ = [("qc_radix", I32), ("qc_cols", I32), ("ggm_dfs", I32), ("lpn_wgs", I32)]
+
+
+class LaunchTime(C.Structure):
+    _fields_ = [("kernel", C.c_char * 32), ("ms", C.c_float), ("bytes", C.c_double)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/cuot_amd.h
 SIGNATURES = {
     "cuot_init": (INT, [INT, C.POINTER(P)]),
@@ -92,6 +110,23 @@ SIGNATURES = {
     "cuot_batch_bind": (INT, [P, P, C.c_size_t, C.POINTER(P), C.POINTER(P), C.POINTER(P), P]),
     "cuot_batch_run": (INT, [P, C.c_uint32, P]),
     "cuot_batch_free": (INT, [P]),
+    "cuot_server_new": (INT, [P, C.POINTER(ServerConfig), C.POINTER(P)]),
+    "cuot_server_submit": (INT, [P, C.POINTER(Instance), C.POINTER(I64)]),
+    "cuot_server_poll": (INT, [P, I64, INT, C.POINTER(ServerResult)]),
+    "cuot_server_release": (INT, [P, I64]),
+    "cuot_server_stats": (INT, [P, C.POINTER(I64), C.POINTER(C.c_size_t)]),
+    "cuot_server_free": (INT, [P]),
+    "cuot_mlp_param_count": (INT, [INT, INT]),
+    "cuot_mlp_workspace_bytes": (C.c_size_t, [INT, INT, I64]),
+    "cuot_mlp_forward": (INT, [P, P, INT, INT, P, I64, P, P]),
+    "cuot_mlp_grad": (INT, [P, P, INT, INT, P, P, I64, P, C.c_size_t, P, P, P]),
+    "cuot_mlp_adam": (INT, [P, P, P, P, P, INT, INT, C.c_float, P]),
+    "cuot_set_knobs": (INT, [C.POINTER(KnobValues)]),
+    "cuot_get_knobs": (INT, [C.POINTER(KnobValues)]),
+    "cuot_timing_begin": (INT, [P]),
+    "cuot_timing_end": (INT, [P, C.POINTER(LaunchTime), INT, C.POINTER(INT)]),
+    "cuot_serve_replay": (INT, [P, C.POINTER(ServerConfig), C.POINTER(Instance), C.POINTER(C.c_double), INT,
+                                C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(I64)]),
 }
 
 _lib = None

@@ -68,7 +68,8 @@ struct cuot_batch {
   char* ws = nullptr;
   std::vector<blk*> out0, out1;
   std::vector<uint32_t*> bits;
-  blk* delta_d = nullptr;
+  std::vector<blk*> delta;  // per-instance Delta destination (may be NULL)
+  std::vector<char> img;    // host image of the launch tables
   bool bound = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -311,9 +312,9 @@ hipError_t launch_all(cuot_batch* b, uint32_t stages, hipStream_t st) {
   if (stages & CUOT_BATCH_STAGE_SETUP) {
     for (int i = 0; i < count; i++) {
       if ((e = b->is_sot(i) ? b->sot[i].run_setup(st) : b->fer[i].run_setup(st))) return e;
-      if (b->delta_d) {
+      if (b->delta[i]) {
         const blk* d = b->is_sot(i) ? b->sot[i].delta : b->fer[i].delta;
-        if ((e = hipMemcpyAsync(b->delta_d + i, d, 16, hipMemcpyDeviceToDevice, st))) return e;
+        if ((e = hipMemcpyAsync(b->delta[i], d, 16, hipMemcpyDeviceToDevice, st))) return e;
       }
     }
   }
@@ -379,8 +380,8 @@ extern "C" int cuot_batch_new(cuot_ctx* ctx, const cuot_instance* inst, int coun
 
 extern "C" size_t cuot_batch_workspace_bytes(const cuot_batch* b) { return b ? b->ws_bytes : 0; }
 
-extern "C" int cuot_batch_bind(cuot_batch* b, void* ws_d, size_t ws_bytes, cuot_block* const* out0,
-                               cuot_block* const* out1, uint8_t* const* bits, cuot_block* delta_d) {
+int cuot_batch_bind_on(cuot_batch* b, void* ws_d, size_t ws_bytes, cuot_block* const* out0, cuot_block* const* out1,
+                       uint8_t* const* bits, cuot_block* const* delta, hipStream_t st) {
   CUOT_CHECK_ARG(b && ws_d && out0 && out1, "NULL argument");
   CUOT_CHECK_ARG(ws_bytes >= b->ws_bytes, "workspace too small (cuot_batch_workspace_bytes)");
   CUOT_CHECK_ARG(((uintptr_t)ws_d & 255) == 0, "workspace must be 256-byte aligned");
@@ -395,24 +396,40 @@ extern "C" int cuot_batch_bind(cuot_batch* b, void* ws_d, size_t ws_bytes, cuot_
   b->out0.assign(count, nullptr);
   b->out1.assign(count, nullptr);
   b->bits.assign(count, nullptr);
+  b->delta.assign(count, nullptr);
   for (int i = 0; i < count; i++) {
     b->out0[i] = (blk*)out0[i];
     b->out1[i] = (blk*)out1[i];
     if (bits) b->bits[i] = (uint32_t*)bits[i];
+    if (delta) b->delta[i] = (blk*)delta[i];
     if (b->is_sot(i))
       b->sot[i].carve(b->ws + b->inst_off[i]);
     else
       b->fer[i].carve(b->ws + b->inst_off[i]);
   }
-  b->delta_d = (blk*)delta_d;
-  std::vector<char> img;
-  int rc = build_tables(b, img);
+  int rc = build_tables(b, b->img);
   if (rc) return rc;
   CUOT_HIP_TRY(hipSetDevice(b->ctx->device));
   CUOT_HIP_TRY(cuot_qc_configure());
-  if (!img.empty()) CUOT_HIP_TRY(hipMemcpy(b->ws, img.data(), img.size(), hipMemcpyHostToDevice));
+  if (!b->img.empty()) {
+    // the image stays alive in the batch until it is freed, so an async copy
+    // from it is safe
+    if (st)
+      CUOT_HIP_TRY(hipMemcpyAsync(b->ws, b->img.data(), b->img.size(), hipMemcpyHostToDevice, st));
+    else
+      CUOT_HIP_TRY(hipMemcpy(b->ws, b->img.data(), b->img.size(), hipMemcpyHostToDevice));
+  }
   b->bound = true;
   return CUOT_OK;
+}
+
+extern "C" int cuot_batch_bind(cuot_batch* b, void* ws_d, size_t ws_bytes, cuot_block* const* out0,
+                               cuot_block* const* out1, uint8_t* const* bits, cuot_block* delta_d) {
+  CUOT_CHECK_ARG(b, "NULL batch");
+  std::vector<cuot_block*> d(b->inst.size(), nullptr);
+  if (delta_d)
+    for (size_t i = 0; i < d.size(); i++) d[i] = delta_d + i;
+  return cuot_batch_bind_on(b, ws_d, ws_bytes, out0, out1, bits, d.data(), nullptr);
 }
 
 extern "C" int cuot_batch_run(cuot_batch* b, uint32_t stages, void* stream) {

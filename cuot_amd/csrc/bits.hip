@@ -167,12 +167,14 @@ hipError_t cuot_launch_transpose_planes2(const void* in, int64_t N, void* out, b
   if (g > cap) g = cap;
   if (g < 1) g = 1;
   dim3 grid((unsigned)g, (unsigned)planes);
+  cuot_rec_begin(st, inverse ? "transpose_inv" : "transpose_fwd", 2.0 * 16.0 * (double)N * planes);
   if (!inverse)
     hipLaunchKernelGGL(transpose_fwd, grid, dim3(256), 0, st, (const uint4*)in, N, (uint32_t*)out, P, in_is, in_ps,
                        out_is, out_ps);
   else
     hipLaunchKernelGGL(transpose_inv, grid, dim3(256), 0, st, (const uint32_t*)in, N, (uint4*)out, P, in_is, in_ps,
                        out_is, out_ps);
+  cuot_rec_end(st);
   return hipGetLastError();
 }
 

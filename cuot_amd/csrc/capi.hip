@@ -249,3 +249,95 @@ extern "C" int cuot_bit_untranspose(cuot_ctx* ctx, const uint8_t* rows_d, int64_
   CUOT_HIP_TRY(cuot_launch_transpose(rows_d, N, out_d, true, ctx->st(stream)));
   return CUOT_OK;
 }
+
+// ---- execution knobs ---------------------------------------------------------
+namespace {
+CuotKnobs g_knobs = {0, 0, 0, 0};
+}
+
+const CuotKnobs& cuot_knobs() { return g_knobs; }
+
+extern "C" int cuot_set_knobs(const cuot_knob_values* k) {
+  CUOT_CHECK_ARG(k, "NULL knobs");
+  CUOT_CHECK_ARG(k->qc_radix == 0 || (k->qc_radix >= 2 && k->qc_radix <= 5), "qc_radix must be 0 or 2..5");
+  CUOT_CHECK_ARG(k->qc_cols == 0 || k->qc_cols == 1 || k->qc_cols == 2 || k->qc_cols == 4 || k->qc_cols == 8,
+                 "qc_cols must be 0, 1, 2, 4 or 8");
+  CUOT_CHECK_ARG(k->ggm_dfs >= 0 && k->ggm_dfs <= 6, "ggm_dfs must be in [0, 6]");
+  CUOT_CHECK_ARG(k->lpn_wgs >= 0, "lpn_wgs must be >= 0");
+  g_knobs.qc_radix = k->qc_radix;
+  g_knobs.qc_cols = k->qc_cols;
+  g_knobs.ggm_dfs = k->ggm_dfs;
+  g_knobs.lpn_wgs = k->lpn_wgs;
+  return CUOT_OK;
+}
+
+extern "C" int cuot_get_knobs(cuot_knob_values* k) {
+  CUOT_CHECK_ARG(k, "NULL knobs");
+  k->qc_radix = g_knobs.qc_radix;
+  k->qc_cols = g_knobs.qc_cols;
+  k->ggm_dfs = g_knobs.ggm_dfs;
+  k->lpn_wgs = g_knobs.lpn_wgs;
+  return CUOT_OK;
+}
+
+// ---- per-launch timing ----------------------------------------------------------
+namespace {
+thread_local LaunchRecorder* t_rec = nullptr;
+}
+
+LaunchRecorder* cuot_recorder() { return t_rec; }
+
+void cuot_rec_begin(hipStream_t st, const char* name, double bytes) {
+  if (!t_rec) return;
+  LaunchRecorder::Item it;
+  it.name = name;
+  it.bytes = bytes;
+  it.a = it.b = nullptr;
+  if (hipEventCreate(&it.a) != hipSuccess || hipEventCreate(&it.b) != hipSuccess) return;
+  hipEventRecord(it.a, st);
+  t_rec->items.push_back(it);
+}
+
+void cuot_rec_end(hipStream_t st) {
+  if (!t_rec || t_rec->items.empty()) return;
+  hipEventRecord(t_rec->items.back().b, st);
+}
+
+extern "C" int cuot_timing_begin(cuot_ctx* ctx) {
+  CUOT_CHECK_ARG(ctx, "NULL ctx");
+  if (t_rec) {
+    cuot_set_last_error("timing already active on this thread", __FILE__, __LINE__);
+    return CUOT_ERR_STATE;
+  }
+  t_rec = new LaunchRecorder();
+  return CUOT_OK;
+}
+
+extern "C" int cuot_timing_end(cuot_ctx* ctx, cuot_launch_time* out, int max_out, int* count) {
+  CUOT_CHECK_ARG(ctx && count && (out || max_out == 0), "NULL argument");
+  if (!t_rec) {
+    cuot_set_last_error("timing not active on this thread", __FILE__, __LINE__);
+    return CUOT_ERR_STATE;
+  }
+  LaunchRecorder* rec = t_rec;
+  t_rec = nullptr;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  int c = 0;
+  for (auto& it : rec->items) {
+    float ms = 0;
+    if (e == hipSuccess && c < max_out && hipEventElapsedTime(&ms, it.a, it.b) == hipSuccess) {
+      memset(out[c].kernel, 0, sizeof(out[c].kernel));
+      strncpy(out[c].kernel, it.name, sizeof(out[c].kernel) - 1);
+      out[c].ms = ms;
+      out[c].bytes = it.bytes;
+      c++;
+    }
+    hipEventDestroy(it.a);
+    hipEventDestroy(it.b);
+  }
+  delete rec;
+  *count = c;
+  CUOT_HIP_TRY(e);
+  return CUOT_OK;
+}

@@ -101,13 +101,15 @@ CB_HD void cb_hi_fix(uint32_t* w, int64_t nw, int k, int64_t tid, int64_t nth) {
   uint32_t e[16];
   const int ne = cb_lower_terms(k, e);
   if (ne == 0) return;
-  const int64_t hw = ((int64_t)1 << k) >> 5;  // words per half
+  const int lhw = k - 5;  // log2 words per half
+  const int64_t hw = (int64_t)1 << lhw;
   const uint32_t emax = e[ne - 1];
-  const int64_t nwr = emax >= 32 ? (int64_t)(emax >> 5) : 1;  // written words per chunk
-  const int64_t nchunk = nw / (2 * hw);
-  for (int64_t t = tid; t < nchunk * nwr; t += nth) {
-    const int64_t g = t / nwr, wi = t % nwr;
-    uint32_t* hi = w + g * 2 * hw + hw;
+  // written words per chunk: emax / 32 (a power of two) or 1
+  const int lnwr = emax >= 32 ? (31 - __builtin_clz(emax)) - 5 : 0;
+  const int64_t total = (nw >> (lhw + 1)) << lnwr;
+  for (int64_t t = tid; t < total; t += nth) {
+    const int64_t g = t >> lnwr, wi = t & (((int64_t)1 << lnwr) - 1);
+    uint32_t* hi = w + (g << (lhw + 1)) + hw;
     uint32_t v = hi[wi];
     uint32_t corr = 0;
     for (int q = 0; q < ne; q++) {
@@ -127,11 +129,12 @@ CB_HD void cb_lo_update(uint32_t* w, int64_t nw, int k, int64_t tid, int64_t nth
   uint32_t e[16];
   const int ne = cb_lower_terms(k, e);
   if (ne == 0) return;
-  const int64_t hw = ((int64_t)1 << k) >> 5;
-  const int64_t nchunk = nw / (2 * hw);
-  for (int64_t t = tid; t < nchunk * hw; t += nth) {
-    const int64_t g = t / hw, wj = t % hw;
-    uint32_t* lo = w + g * 2 * hw;
+  const int lhw = k - 5;
+  const int64_t hw = (int64_t)1 << lhw;
+  const int64_t total = nw >> 1;  // lo words of all chunks
+  for (int64_t t = tid; t < total; t += nth) {
+    const int64_t g = t >> lhw, wj = t & (hw - 1);
+    uint32_t* lo = w + (g << (lhw + 1));
     const uint32_t* hi = lo + hw;
     uint32_t v = lo[wj];
     for (int q = 0; q < ne; q++) {

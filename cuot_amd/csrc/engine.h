@@ -46,3 +46,8 @@ struct FerretPlan {
   void lpn_launch(blk* out_s, blk* out_r, LpnLaunch& l) const;
   hipError_t run_lpn(blk* out_s, blk* out_r, hipStream_t st);
 };
+
+// Batch binding with an explicit stream for the table upload (NULL: synchronous)
+// and per-instance Delta destinations; the public cuot_batch_bind wraps it.
+int cuot_batch_bind_on(cuot_batch* b, void* ws_d, size_t ws_bytes, cuot_block* const* out0, cuot_block* const* out1,
+                       uint8_t* const* bits, cuot_block* const* delta, hipStream_t st);

@@ -277,12 +277,20 @@ __global__ __launch_bounds__(WG) void ggm_batch_kernel(const GgmKArgs* __restric
   ggm_tree<L, RECV>(tab[lo], s, g - start[lo]);
 }
 
-int ggm_split(int h) { return h <= 8 ? 0 : (h - 8 < LMAX ? h - 8 : LMAX); }
+int ggm_split(int h) {
+  if (h <= 8) return 0;
+  const int k = cuot_knobs().ggm_dfs;
+  const int hi = h - 8 < LMAX ? h - 8 : LMAX;
+  if (k >= 1 && k <= hi && h - k <= MAXTOP) return k;
+  return hi;
+}
 
 template <bool RECV>
 hipError_t launch_ggm(const GgmKArgs& a, hipStream_t st) {
   const int L = ggm_split(a.h);
   dim3 grid((unsigned)a.t), block(WG);
+  const int64_t nl = (a.t << a.h) < a.n_out ? (a.t << a.h) : a.n_out;
+  cuot_rec_begin(st, RECV ? "ggm_kernel<recv>" : "ggm_kernel<send>", 16.0 * (double)nl);
   switch (L) {
 #define CUOT_GGM_CASE(X) \
   case X:                \
@@ -297,8 +305,10 @@ hipError_t launch_ggm(const GgmKArgs& a, hipStream_t st) {
     CUOT_GGM_CASE(6)
 #undef CUOT_GGM_CASE
     default:
+      cuot_rec_end(st);
       return hipErrorInvalidValue;
   }
+  cuot_rec_end(st);
   return hipGetLastError();
 }
 

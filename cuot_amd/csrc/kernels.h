@@ -1,6 +1,8 @@
 // kernels.h -- internal kernel argument structs and launchers (not ABI).
 #pragma once
 
+#include <vector>
+
 #include "common.h"
 
 // GGM expansion of t trees of depth h (one 256-lane workgroup per tree).
@@ -92,6 +94,32 @@ hipError_t cuot_launch_transpose_planes(const void* in, int64_t N, void* out, bo
 hipError_t cuot_launch_transpose_planes2(const void* in, int64_t N, void* out, bool inverse, int planes, int P,
                                         int64_t in_is, int64_t in_ps, int64_t out_is, int64_t out_ps,
                                         hipStream_t st);
+
+// Per-launch timing (capi.hip): when a recorder is installed on the calling
+// thread, instrumented launch sites bracket their kernel with HIP events on
+// the launch stream and note the kernel's algorithmic bytes.
+struct LaunchRecorder {
+  struct Item {
+    const char* name;
+    hipEvent_t a, b;
+    double bytes;
+  };
+  std::vector<Item> items;
+  hipStream_t st = nullptr;
+};
+LaunchRecorder* cuot_recorder();
+void cuot_rec_begin(hipStream_t st, const char* name, double bytes);
+void cuot_rec_end(hipStream_t st);
+
+// Execution knobs (capi.hip): process-wide, picked by the autotuner; 0 means
+// the built-in rule.  Knobs never change outputs.
+struct CuotKnobs {
+  int qc_radix;  // FFT layers per upper pass (2..5)
+  int qc_cols;   // FFT columns per tile (1, 2, 4, 8)
+  int ggm_dfs;   // per-lane depth-first levels L (h - 10 <= L <= min(h - 8, 6))
+  int lpn_wgs;   // LPN grid cap
+};
+const CuotKnobs& cuot_knobs();
 
 // Protocol glue (proto.hip).
 hipError_t cuot_launch_setup_fix(blk* delta, const blk* alpha_blocks, int64_t t, int h, uint8_t* choice,

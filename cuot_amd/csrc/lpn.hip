@@ -182,12 +182,16 @@ hipError_t cuot_launch_lpn(const LpnLaunch& l, hipStream_t st) {
   LpnKArgs a;
   if (!fill_args(l, a)) return hipErrorInvalidValue;
   int64_t g = (a.units_total + 255) / 256;
-  if (g > 8192) g = 8192;
+  const int64_t cap = cuot_knobs().lpn_wgs > 0 ? cuot_knobs().lpn_wgs : 8192;
+  if (g > cap) g = cap;
   if (g < 1) g = 1;
+  // algorithmic bytes per role and row: noise in, d gathers, row out
+  cuot_rec_begin(st, "lpn_kernel", 16.0 * (D + 2) * (double)l.n * l.nroles);
   if (l.nroles == 2)
     hipLaunchKernelGGL(lpn_kernel<2>, dim3((unsigned)g), dim3(256), 0, st, a);
   else
     hipLaunchKernelGGL(lpn_kernel<1>, dim3((unsigned)g), dim3(256), 0, st, a);
+  cuot_rec_end(st);
   return hipGetLastError();
 }
 

@@ -39,6 +39,8 @@ constexpr int CONV_G = 4;      // planes per conversion group (ping-pong buffer 
 constexpr int FFT_TC = 256;    // elements per FFT LDS tile (2^R * C)
 constexpr int FFT_STRIDE = 132;  // LDS words per element (32 planes x 4 words + pad)
 constexpr int FFT_THREADS = 512;
+constexpr int FFT_LD = 16;  // 16-byte loads in flight per lane (tile staging)
+constexpr int CR_LD = 8;    // same for the bit-row conversion tiles
 // the row-layout conversion handles strings of 2^5 .. 2^20 coefficients (128 KB LDS)
 constexpr int ROW_MP_MIN = 5, ROW_MP_MAX = 20;
 
@@ -63,6 +65,24 @@ CUOT_DEV uint32_t chunk_const(const uint32_t* beta, uint64_t g) {
   for (int b = 0; g; b++, g >>= 1)
     if (g & 1) c ^= beta[b + 1];
   return c;
+}
+
+// C(g) from three 256-entry tables of the low 24 bits of g (tab[256*q + v] =
+// C(v << 8q)); g < 2^24 for every transform length handled here (m <= 24)
+constexpr int CTAB_WORDS = 3 * 256;
+
+CUOT_DEV void ctab_fill(uint32_t* tab, int tid, int nth) {
+  for (int i = tid; i < CTAB_WORDS; i += nth) {
+    const int q = i >> 8, v = i & 255;
+    uint32_t c = 0;
+    for (int b = 0; b < 8; b++)
+      if ((v >> b) & 1) c ^= c_beta[8 * q + b + 1];
+    tab[i] = c;
+  }
+}
+
+CUOT_DEV uint32_t ctab_const(const uint32_t* tab, uint32_t g) {
+  return tab[g & 255] ^ tab[256 + ((g >> 8) & 255)] ^ tab[512 + ((g >> 16) & 255)];
 }
 
 CUOT_DEV uint4 x4(uint4 a, uint4 b) { return make_uint4(a.x ^ b.x, a.y ^ b.y, a.z ^ b.z, a.w ^ b.w); }
@@ -218,10 +238,36 @@ __global__ __launch_bounds__(1024) void qc_conv_rows(uint32_t* __restrict__ Bw, 
   extern __shared__ uint32_t sw[];
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t* g = Bw + (tile / tiles_per_inst) * inst_words + (tile % tiles_per_inst) * tile_words;
-    for (int64_t i = threadIdx.x; i < tile_words; i += blockDim.x) sw[i] = g[i];
+    if ((tile_words & 3) == 0) {
+      // 16-byte accesses, CR_LD in flight per lane
+      const int n4 = (int)(tile_words >> 2);
+      const uint4* g4 = (const uint4*)g;
+      for (int f0 = threadIdx.x; f0 < n4; f0 += blockDim.x * CR_LD) {
+        uint4 v[CR_LD];
+#pragma unroll
+        for (int u = 0; u < CR_LD; u++) {
+          const int f = f0 + u * (int)blockDim.x;
+          v[u] = z4();
+          if (f < n4) v[u] = g4[f];
+        }
+#pragma unroll
+        for (int u = 0; u < CR_LD; u++) {
+          const int f = f0 + u * (int)blockDim.x;
+          if (f < n4) ((uint4*)sw)[f] = v[u];
+        }
+      }
+    } else {
+      for (int64_t i = threadIdx.x; i < tile_words; i += blockDim.x) sw[i] = g[i];
+    }
     __syncthreads();
     cb_convert(sw, mp, inverse != 0, threadIdx.x, blockDim.x, BarrierSync());
-    for (int64_t i = threadIdx.x; i < tile_words; i += blockDim.x) g[i] = sw[i];
+    if ((tile_words & 3) == 0) {
+      const int n4 = (int)(tile_words >> 2);
+      uint4* g4 = (uint4*)g;
+      for (int f = threadIdx.x; f < n4; f += blockDim.x) g4[f] = ((const uint4*)sw)[f];
+    } else {
+      for (int64_t i = threadIdx.x; i < tile_words; i += blockDim.x) g[i] = sw[i];
+    }
     __syncthreads();
   }
 }
@@ -246,6 +292,7 @@ struct FftPassArgs {
 
 __device__ __attribute__((noinline)) void fft_layer(int lo, int R, int C, int lgC, uint32_t* lds, int64_t hib,
                                                   int64_t cb, int k, bool inv) {
+  const uint32_t* ctab = lds + (C << R) * FFT_STRIDE;
   const int s = k - lo;
   const int T = 1 << R;
   const int ntask = (T >> 1) * C * 4;
@@ -265,17 +312,17 @@ __device__ __attribute__((noinline)) void fft_layer(int lo, int R, int C, int lg
       xu[p] = pu[4 * p];
       xv[p] = pv[4 * p];
     }
-    const uint64_t g0 = __builtin_amdgcn_readfirstlane((uint32_t)g) | ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(g >> 32)) << 32);
-    const bool uni = __all(g == g0);
+    const uint32_t g0 = __builtin_amdgcn_readfirstlane((uint32_t)g);
+    const bool uni = __all((uint32_t)g == g0);
     if (inv) {
 #pragma unroll
       for (int p = 0; p < 32; p++) xv[p] ^= xu[p];
     }
     if (uni) {
-      const uint32_t cst = __builtin_amdgcn_readfirstlane(chunk_const(c_beta, g0));
+      const uint32_t cst = __builtin_amdgcn_readfirstlane(ctab_const(ctab, g0));
       gf32_mulc_acc<true>(xv, cst, xu);
     } else {
-      gf32_mulc_acc<false>(xv, chunk_const(c_beta, g), xu);
+      gf32_mulc_acc<false>(xv, ctab_const(ctab, (uint32_t)g), xu);
     }
     if (!inv) {
 #pragma unroll
@@ -320,22 +367,37 @@ __global__ __launch_bounds__(FFT_THREADS) void qc_fft_pass(FftPassArgs a) {
   const int64_t cblocks = ((int64_t)1 << a.lo) >> a.lgC;
   const int64_t ntiles = a.M / TC;
   const int64_t half = a.M >> 1;
+  ctab_fill(lds + TC * FFT_STRIDE, threadIdx.x, blockDim.x);  // read after the first barrier
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t cb = tile & (cblocks - 1), hib = tile >> (a.lo - a.lgC);
-    // load: f = ((p*T + t)*C + c)*4 + w  (w, c, t fastest: contiguous runs)
-    const int nf = TC * 32 * 4;
-    for (int f = threadIdx.x; f < nf; f += blockDim.x) {
-      const int w = f & 3;
-      const int c = (f >> 2) & (a.C - 1);
-      const int pt = (f >> 2) >> a.lgC;
-      const int t = pt & (T - 1), p = pt >> a.R;
-      const int64_t idx = (hib << (a.lo + a.R)) | ((int64_t)t << a.lo) | (cb * a.C + c);
-      uint32_t v;
-      if (a.first)
-        v = (idx < half && p < 16) ? a.X[((int64_t)p * a.M + idx) * 4 + w] : 0u;
-      else
-        v = a.X[((int64_t)p * a.M + idx) * 4 + w];
-      lds[(t * a.C + c) * FFT_STRIDE + p * 4 + w] = v;
+    // load: one 16-byte plane word group per lane, f = (p*T + t)*C + c
+    // (c, t fastest: contiguous runs); FFT_LD loads in flight per lane
+    const int n4 = TC * 32;
+    const uint4* X4 = (const uint4*)a.X;
+    for (int f0 = threadIdx.x; f0 < n4; f0 += blockDim.x * FFT_LD) {
+      uint4 v[FFT_LD];
+#pragma unroll
+      for (int u = 0; u < FFT_LD; u++) {
+        const int f = f0 + u * (int)blockDim.x;
+        v[u] = z4();
+        if (f < n4) {
+          const int c = f & (a.C - 1);
+          const int pt = f >> a.lgC;
+          const int t = pt & (T - 1), p = pt >> a.R;
+          const int64_t idx = (hib << (a.lo + a.R)) | ((int64_t)t << a.lo) | (cb * a.C + c);
+          if (!a.first || (idx < half && p < 16)) v[u] = X4[(int64_t)p * a.M + idx];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < FFT_LD; u++) {
+        const int f = f0 + u * (int)blockDim.x;
+        if (f < n4) {
+          const int c = f & (a.C - 1);
+          const int pt = f >> a.lgC;
+          const int t = pt & (T - 1), p = pt >> a.R;
+          *(uint4*)&lds[(t * a.C + c) * FFT_STRIDE + p * 4] = v[u];
+        }
+      }
     }
     __syncthreads();
     if (a.mode == PASS_FWD || a.mode == PASS_FUSED) {
@@ -345,13 +407,19 @@ __global__ __launch_bounds__(FFT_THREADS) void qc_fft_pass(FftPassArgs a) {
     if (a.mode == PASS_INV || a.mode == PASS_FUSED) {
       for (int k = a.lo; k <= a.khi; k++) fft_layer(a.lo, a.R, a.C, a.lgC, lds, hib, cb, k, true);
     }
-    for (int f = threadIdx.x; f < nf; f += blockDim.x) {
-      const int w = f & 3;
-      const int c = (f >> 2) & (a.C - 1);
-      const int pt = (f >> 2) >> a.lgC;
-      const int t = pt & (T - 1), p = pt >> a.R;
-      const int64_t idx = (hib << (a.lo + a.R)) | ((int64_t)t << a.lo) | (cb * a.C + c);
-      a.X[((int64_t)p * a.M + idx) * 4 + w] = lds[(t * a.C + c) * FFT_STRIDE + p * 4 + w];
+    uint4* Y4 = (uint4*)a.X;
+    for (int f0 = threadIdx.x; f0 < n4; f0 += blockDim.x * FFT_LD) {
+#pragma unroll
+      for (int u = 0; u < FFT_LD; u++) {
+        const int f = f0 + u * (int)blockDim.x;
+        if (f < n4) {
+          const int c = f & (a.C - 1);
+          const int pt = f >> a.lgC;
+          const int t = pt & (T - 1), p = pt >> a.R;
+          const int64_t idx = (hib << (a.lo + a.R)) | ((int64_t)t << a.lo) | (cb * a.C + c);
+          Y4[(int64_t)p * a.M + idx] = *(const uint4*)&lds[(t * a.C + c) * FFT_STRIDE + p * 4];
+        }
+      }
     }
     __syncthreads();
   }
@@ -524,7 +592,7 @@ hipError_t configure_lds() {
     if (e0 != hipSuccess) return e0;
   }
   hipError_t e = hipFuncSetAttribute((const void*)qc_fft_pass, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     FFT_TC * FFT_STRIDE * 4);
+                                     (FFT_TC * FFT_STRIDE + CTAB_WORDS) * 4);
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute((const void*)qc_conv_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
                           ((int64_t)1 << ROW_MP_MAX) / 8);
@@ -617,8 +685,10 @@ hipError_t conv_forward(Bufs b, int64_t M, int P, int mp, hipStream_t st) {
                                                128 * row_words, st);
   if (e != hipSuccess) return e;
   const int64_t tpi = (int64_t)P * 128;
-  hipLaunchKernelGGL(qc_conv_rows, dim3(capg(tpi * b.B, 4096)), dim3(1024), (size_t)row_words * 4, st, b.Bw,
+  cuot_rec_begin(st, "qc_conv_rows", 2.0 * (double)tpi * b.B * row_words * 4);
+    hipLaunchKernelGGL(qc_conv_rows, dim3(capg(tpi * b.B, 4096)), dim3(1024), (size_t)row_words * 4, st, b.Bw,
                      row_words, tpi * b.B, tpi, inst_words, mp, 0);
+  cuot_rec_end(st);
   return cuot_launch_transpose_planes2(b.Bw, Mp, b.X, true, b.B * P, P, inst_words, 128 * row_words, b.XS, M, st);
 }
 
@@ -633,8 +703,10 @@ hipError_t conv_inverse(Bufs b, int64_t M, int P, int mtop, hipStream_t st) {
   if (e != hipSuccess) return e;
   const int64_t tile_words = row_words / 2;
   const int64_t tpi = (int64_t)P * 128 * 2;
-  hipLaunchKernelGGL(qc_conv_rows, dim3(capg(tpi * b.B, 8192)), dim3(1024), (size_t)tile_words * 4, st, b.Bw,
+  cuot_rec_begin(st, "qc_conv_rows", 2.0 * (double)tpi * b.B * tile_words * 4);
+    hipLaunchKernelGGL(qc_conv_rows, dim3(capg(tpi * b.B, 8192)), dim3(1024), (size_t)tile_words * 4, st, b.Bw,
                      tile_words, tpi * b.B, tpi, inst_words, mtop, 1);
+  cuot_rec_end(st);
   return cuot_launch_transpose_planes2(b.Bw, M, b.X, true, b.B * P, P, inst_words, 128 * row_words, b.XS, M, st);
 }
 
@@ -661,8 +733,14 @@ FftPassArgs pass_args(const Bufs& b, int64_t M, int lo, int R, int C, int mode, 
 void launch_pass(const FftPassArgs& a, int B, hipStream_t st) {
   unsigned g;
   pass_grid(a.M / ((1 << a.R) * a.C), g);
-  hipLaunchKernelGGL(qc_fft_pass, dim3(g, (unsigned)B), dim3(FFT_THREADS), (size_t)(1 << a.R) * a.C * FFT_STRIDE * 4,
+  // algorithmic bytes: every element's 32 planes read and written once (the
+  // first pass reads only planes 0..15 of the lower half)
+  const double per = (double)a.M * 512.0 * B;
+  cuot_rec_begin(st, "qc_fft_pass", a.first ? per * 1.25 : per * 2.0);
+  hipLaunchKernelGGL(qc_fft_pass, dim3(g, (unsigned)B), dim3(FFT_THREADS),
+                     ((size_t)(1 << a.R) * a.C * FFT_STRIDE + CTAB_WORDS) * 4,
                      st, a);
+  cuot_rec_end(st);
 }
 
 // forward FFT layers m-1..0 (top layer from the packed lower half); with A^,
@@ -674,10 +752,12 @@ hipError_t fft_forward(const Bufs& b, int m, const uint32_t* ahat1, const uint32
   const int R0 = m < 8 ? m : 8;
   int khi = m - 1;
   bool first = true;
+  const int RMAX = cuot_knobs().qc_radix > 0 ? cuot_knobs().qc_radix : 5;
+  const int CMAX = cuot_knobs().qc_cols > 0 ? cuot_knobs().qc_cols : 8;
   while (khi >= R0) {
-    int R = khi - R0 + 1 < 5 ? khi - R0 + 1 : 5;
+    int R = khi - R0 + 1 < RMAX ? khi - R0 + 1 : RMAX;
     int lo = khi - R + 1;
-    int C = 8;
+    int C = CMAX;
     while (C > 1 && (((int64_t)1 << lo) % C)) C >>= 1;
     while (((1 << R) * C) > FFT_TC) C >>= 1;
     launch_pass(pass_args(b, M, lo, R, C, PASS_FWD, khi, first ? 1 : 0, nullptr, nullptr), b.B, st);
@@ -692,9 +772,11 @@ hipError_t fft_forward(const Bufs& b, int m, const uint32_t* ahat1, const uint32
 hipError_t fft_inverse_upper(const Bufs& b, int m, int from, hipStream_t st) {
   const int64_t M = (int64_t)1 << m;
   int klo = from;
+  const int RMAX = cuot_knobs().qc_radix > 0 ? cuot_knobs().qc_radix : 5;
+  const int CMAX = cuot_knobs().qc_cols > 0 ? cuot_knobs().qc_cols : 8;
   while (klo <= m - 1) {
-    int R = m - klo < 5 ? m - klo : 5;
-    int C = 8;
+    int R = m - klo < RMAX ? m - klo : RMAX;
+    int C = CMAX;
     while (C > 1 && (((int64_t)1 << klo) % C)) C >>= 1;
     while (((1 << R) * C) > FFT_TC) C >>= 1;
     launch_pass(pass_args(b, M, klo, R, C, PASS_INV, klo + R - 1, 0, nullptr, nullptr), b.B, st);
@@ -710,16 +792,20 @@ hipError_t encode_batch(const QcIo* io, QcIo io1, const uint32_t* ahat1, const u
   if (B > 1 && !row_path(m - 1)) return hipErrorInvalidValue;
   const int64_t M = (int64_t)1 << m;
   Bufs b = bufs(scratch, M, B);
-  hipLaunchKernelGGL(qc_pack_x, dim3(grid_for(16 * (M >> 1)), (unsigned)B), dim3(256), 0, st, io, io1, (uint4*)b.X, M,
+  cuot_rec_begin(st, "qc_pack_x", 2.0 * 16.0 * (double)(M >> 1) * 16 * B);
+    hipLaunchKernelGGL(qc_pack_x, dim3(grid_for(16 * (M >> 1)), (unsigned)B), dim3(256), 0, st, io, io1, (uint4*)b.X, M,
                      b.XS);
+  cuot_rec_end(st);
   if ((e = conv_forward(b, M, 16, m - 1, st))) return e;
   int khi;
   if ((e = fft_forward(b, m, ahat1, ahats, true, &khi, st))) return e;
   if ((e = fft_inverse_upper(b, m, khi + 1, st))) return e;
   if ((e = conv_inverse(b, M, 31, m - 1, st))) return e;
   const int64_t nmax = io ? (M / 2) * 16 : io1.n;  // grid covers the largest n of the class
-  hipLaunchKernelGGL(qc_output, dim3(grid_for(nmax), (unsigned)B), dim3(256), 0, st, (const uint4*)b.X, M, m, b.XS,
+  cuot_rec_begin(st, "qc_output", 32.0 * (double)nmax * B);
+    hipLaunchKernelGGL(qc_output, dim3(grid_for(nmax), (unsigned)B), dim3(256), 0, st, (const uint4*)b.X, M, m, b.XS,
                      io, io1);
+  cuot_rec_end(st);
   return hipGetLastError();
 }
 

@@ -49,6 +49,9 @@ def _torch_view(ptr: int, nbytes: int, device=0):
     return torch.as_tensor(h, device=f"cuda:{device}")
 
 
+device_bytes = _torch_view
+
+
 class _Role:
     _new = None
 

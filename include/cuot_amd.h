@@ -313,6 +313,90 @@ int cuot_batch_bind(cuot_batch *b, void *ws_d, size_t ws_bytes, cuot_block *cons
 int cuot_batch_run(cuot_batch *b, uint32_t stages, void *stream);
 int cuot_batch_free(cuot_batch *b);
 
+/* ---- serving --------------------------------------------------------------------
+ * No counterpart in the reference.  A request loop that admits correlation
+ * requests (cuot_instance records) against an HBM budget and runs them with
+ * continuous batching on several streams: whenever a stream and arena memory
+ * are free, the queued requests that fit (arrival order) become one batch.
+ * Results are byte-identical to the plain run of each request alone.  With
+ * plain = 1 the loop is the FIFO comparator: one stream, one request at a time
+ * through cuot_sot_run / cuot_ferret_run. */
+typedef struct {
+  size_t hbm_budget;     /* bytes of the device arena (outputs + workspaces) */
+  int32_t max_batch;     /* requests per batch (0 = 64) */
+  int32_t streams;       /* concurrent batches (0 = 2) */
+  size_t qc_chunk_bytes; /* cuot_batch_new knob (0 = default) */
+  int32_t use_graph;     /* replay batches through captured graphs */
+  int32_t plain;         /* FIFO plain comparator */
+} cuot_server_config;
+
+typedef struct {
+  int32_t done;
+  int32_t status;
+  double t_submit, t_start, t_done; /* host steady-clock seconds */
+  cuot_block *out0, *out1;          /* device, valid until release */
+  uint8_t *bits;                    /* SOT choice bits (NULL for Ferret) */
+  cuot_block *delta;                /* device, one block */
+} cuot_server_result;
+
+typedef struct cuot_server cuot_server;
+int cuot_server_new(cuot_ctx *ctx, const cuot_server_config *cfg, cuot_server **out);
+int cuot_server_submit(cuot_server *s, const cuot_instance *inst, int64_t *ticket);
+/* wait != 0 blocks until the request is done */
+int cuot_server_poll(cuot_server *s, int64_t ticket, int wait, cuot_server_result *res);
+int cuot_server_release(cuot_server *s, int64_t ticket);
+int cuot_server_stats(cuot_server *s, int64_t *batches, size_t *arena_used);
+int cuot_server_free(cuot_server *s);
+/* Real-time replay of an arrival schedule through a fresh server; latency_s[i]
+ * = completion - scheduled arrival of request i. */
+int cuot_serve_replay(cuot_ctx *ctx, const cuot_server_config *cfg, const cuot_instance *reqs,
+                      const double *arrival_s, int count, double *latency_s, double *wall_s, int64_t *batches);
+
+/* ---- autotuner cost model ---------------------------------------------------------
+ * No counterpart in the reference.  A small network (F inputs -> H tanh ->
+ * H tanh -> 1) that predicts log kernel time from instance sizes and execution
+ * knobs; trained on measured timings with these kernels (mean squared error,
+ * Adam), used only to pick knobs, never on the data path.  Parameters are one
+ * float32 vector: W1[H][F], b1[H], W2[H][H], b2[H], w3[H], b3. */
+int cuot_mlp_param_count(int F, int H); /* -1 if F > 32 or H > 64 */
+size_t cuot_mlp_workspace_bytes(int F, int H, int64_t N);
+int cuot_mlp_forward(cuot_ctx *ctx, const float *params_d, int F, int H, const float *x_d, int64_t N, float *y_d,
+                     void *stream);
+/* grad_d[P] = d loss / d params, loss_d[1] = mean (y - target)^2 (deterministic
+ * summation order) */
+int cuot_mlp_grad(cuot_ctx *ctx, const float *params_d, int F, int H, const float *x_d, const float *target_d,
+                  int64_t N, void *ws_d, size_t ws_bytes, float *grad_d, float *loss_d, void *stream);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8); step counts from 1 */
+int cuot_mlp_adam(cuot_ctx *ctx, float *params_d, const float *grad_d, float *m_d, float *v_d, int P, int step,
+                  float lr, void *stream);
+
+/* ---- execution knobs ---------------------------------------------------------------
+ * No counterpart in the reference.  Process-wide execution parameters picked
+ * by the autotuner (cuot_amd.tune); 0 selects the built-in rule.  Knobs never
+ * change outputs.  Set them while no work is being launched. */
+typedef struct {
+  int32_t qc_radix; /* FFT layers per upper pass, 2..5 */
+  int32_t qc_cols;  /* FFT columns per tile: 1, 2, 4 or 8 */
+  int32_t ggm_dfs;  /* GGM per-lane depth-first levels (valid: h-10 <= L <= min(h-8, 6); else the rule) */
+  int32_t lpn_wgs;  /* LPN grid cap */
+} cuot_knob_values;
+int cuot_set_knobs(const cuot_knob_values *k);
+int cuot_get_knobs(cuot_knob_values *k);
+
+/* ---- per-kernel timing ----------------------------------------------------------------
+ * No counterpart in the reference.  Between cuot_timing_begin and
+ * cuot_timing_end, every instrumented kernel launched from the calling thread
+ * is bracketed by HIP events on its launch stream; end synchronizes the
+ * device and returns, per launch, the kernel name, its duration and its
+ * algorithmic HBM bytes (for roofline reporting). */
+typedef struct {
+  char kernel[32];
+  float ms;
+  double bytes;
+} cuot_launch_time;
+int cuot_timing_begin(cuot_ctx *ctx);
+int cuot_timing_end(cuot_ctx *ctx, cuot_launch_time *out, int max_out, int *count);
+
 #ifdef __cplusplus
 }
 #endif

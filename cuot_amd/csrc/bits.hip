@@ -18,7 +18,7 @@ CUOT_DEV uint32_t cyc_window(const uint32_t* __restrict__ a, int64_t n, int64_t 
   uint32_t out = 0;
   int got = 0;
   while (got < 32) {
-    const int64_t s = start % n;
+    const int64_t s = start >= n ? start - n : start;  // start < 2n on every call
     int64_t avail = n - s;  // bits before wrap
     int take = (int)((32 - got) < avail ? (32 - got) : avail);
     const int64_t wi = s >> 5;
@@ -46,7 +46,7 @@ __global__ void sparse_encode_kernel(const uint32_t* __restrict__ a, int64_t n, 
         if ((p >> 5) == W) acc ^= 1u << (p & 31);
       } else {
         const int64_t s = p - n;  // out_i ^= a_{(i - s) mod n}
-        int64_t start = (lo - s) % n;
+        int64_t start = lo - s;  // in (-n, n): lo < n and 0 <= s < n
         if (start < 0) start += n;
         acc ^= cyc_window(a, n, start);
       }

@@ -1,0 +1,214 @@
+// conv_fast.h -- the bit-row basis conversion of conv_bits.h with every level
+// specialised at compile time: the subspace-polynomial terms of level k
+// (e = 2^j, j a proper submask of k) become immediate word offsets and shift
+// counts, so the per-word loops carry no term tables.  Same arguments and
+// results as cb_convert (the host test compares the two on every level).
+#pragma once
+
+#include "conv_bits.h"
+
+template <int J>
+CB_HD uint32_t cbt_lo_term(const uint32_t* hi, int64_t wj) {
+  if constexpr (J >= 5) {
+    constexpr int64_t qw = (int64_t)1 << (J - 5);
+    return wj >= qw ? hi[wj - qw] : 0u;
+  } else {
+    constexpr int r = 1 << J;
+    const uint32_t a = hi[wj];
+    const uint32_t b = wj >= 1 ? hi[wj - 1] : 0u;
+    return (a << r) | (b >> (32 - r));
+  }
+}
+
+template <int K, int J = 0>
+CB_HD void cbt_lo_terms(uint32_t& v, const uint32_t* hi, int64_t wj) {
+  if constexpr (J < K) {
+    if constexpr ((J & ~K) == 0) v ^= cbt_lo_term<J>(hi, wj);
+    cbt_lo_terms<K, J + 1>(v, hi, wj);
+  }
+}
+
+// largest proper submask exponent of K (-1 if none)
+template <int K>
+constexpr int cbt_jmax() {
+  int m = -1;
+  for (int j = 0; j < K; j++)
+    if ((j & ~K) == 0) m = j;
+  return m;
+}
+
+template <int K, int J = 0>
+CB_HD void cbt_hi_terms(uint32_t& corr, const uint32_t* hi, int64_t hw, int64_t wi) {
+  if constexpr (J < K) {
+    if constexpr ((J & ~K) == 0) {
+      if constexpr (J >= 5) {
+        constexpr int64_t ew = (int64_t)1 << (J - 5);
+        if (wi < ew) corr ^= hi[wi + hw - ew];
+      } else {
+        if (wi == 0) corr ^= hi[hw - 1] >> (32 - (1 << J));
+      }
+    }
+    cbt_hi_terms<K, J + 1>(corr, hi, hw, wi);
+  }
+}
+
+template <int K>
+CB_HD void cbt_hi_fix(uint32_t* w, int64_t nw, int64_t tid, int64_t nth) {
+  constexpr int JM = cbt_jmax<K>();
+  if constexpr (JM < 0) {
+    return;
+  } else {
+    constexpr int lhw = K - 5;
+    constexpr int64_t hw = (int64_t)1 << lhw;
+    constexpr int lnwr = JM >= 5 ? JM - 5 : 0;  // written words per chunk (log2)
+    const int64_t total = (nw >> (lhw + 1)) << lnwr;
+    for (int64_t t = tid; t < total; t += nth) {
+      const int64_t g = t >> lnwr, wi = t & (((int64_t)1 << lnwr) - 1);
+      uint32_t* hi = w + (g << (lhw + 1)) + hw;
+      uint32_t corr = 0;
+      cbt_hi_terms<K>(corr, hi, hw, wi);
+      hi[wi] ^= corr;
+    }
+  }
+}
+
+template <int K>
+CB_HD void cbt_lo_update(uint32_t* w, int64_t nw, int64_t tid, int64_t nth) {
+  if constexpr (cbt_jmax<K>() < 0) {
+    return;
+  } else {
+    constexpr int lhw = K - 5;
+    constexpr int64_t hw = (int64_t)1 << lhw;
+    const int64_t total = nw >> 1;
+    for (int64_t t = tid; t < total; t += nth) {
+      const int64_t g = t >> lhw, wj = t & (hw - 1);
+      uint32_t* lo = w + (g << (lhw + 1));
+      uint32_t v = lo[wj];
+      cbt_lo_terms<K>(v, lo + hw, wj);
+      lo[wj] = v;
+    }
+  }
+}
+
+// ---- in-word levels (k <= 4) with constant masks and shifts ----------------
+constexpr uint32_t cbt_mask_lo(int b) {
+  return b == 0 ? 0x55555555u : b == 1 ? 0x33333333u : b == 2 ? 0x0F0F0F0Fu : b == 3 ? 0x00FF00FFu : 0x0000FFFFu;
+}
+
+constexpr uint32_t cbt_mask_below(int j, int k) {
+  uint32_t m = 0xFFFFFFFFu;
+  for (int b = j; b <= k; b++) m &= cbt_mask_lo(b);
+  return m;
+}
+
+template <int K, int J = 0>
+CB_HD uint32_t cbt_word_corr(uint32_t h) {
+  if constexpr (J >= K) {
+    return 0u;
+  } else {
+    uint32_t c = cbt_word_corr<K, J + 1>(h);
+    if constexpr ((J & ~K) == 0) c ^= (h >> ((1 << K) - (1 << J))) & cbt_mask_below(J, K);
+    return c;
+  }
+}
+
+template <int K, int J = 0>
+CB_HD uint32_t cbt_word_lo(uint32_t h) {
+  if constexpr (J >= K) {
+    return 0u;
+  } else {
+    uint32_t c = cbt_word_lo<K, J + 1>(h);
+    if constexpr ((J & ~K) == 0) c ^= (h << (1 << J)) & cbt_mask_lo(K);
+    return c;
+  }
+}
+
+template <int K, bool INV>
+CB_HD uint32_t cbt_word_level(uint32_t x) {
+  if constexpr (K == 0) {
+    return x;  // s_0(x) = x^1: no lower terms
+  } else {
+    constexpr uint32_t mlo = cbt_mask_lo(K);
+    uint32_t h = (x >> (1 << K)) & mlo;
+    uint32_t l = x & mlo;
+    if constexpr (!INV) {
+      h ^= cbt_word_corr<K>(h);
+      l ^= cbt_word_lo<K>(h);
+    } else {
+      l ^= cbt_word_lo<K>(h);
+      h ^= cbt_word_corr<K>(h);
+    }
+    return l | (h << (1 << K));
+  }
+}
+
+// in-word levels 4..0 (forward) or 0..4 (inverse) of every word; needs mp >= 5
+CB_HD void cbt_word_levels(uint32_t* w, int64_t nw, bool inverse, int64_t tid, int64_t nth) {
+  for (int64_t i = tid; i < nw; i += nth) {
+    uint32_t x = w[i];
+    if (!inverse) {
+      x = cbt_word_level<4, false>(x);
+      x = cbt_word_level<3, false>(x);
+      x = cbt_word_level<2, false>(x);
+      x = cbt_word_level<1, false>(x);
+    } else {
+      x = cbt_word_level<1, true>(x);
+      x = cbt_word_level<2, true>(x);
+      x = cbt_word_level<3, true>(x);
+      x = cbt_word_level<4, true>(x);
+    }
+    w[i] = x;
+  }
+}
+
+template <typename Sync>
+CB_HD void cbt_level(uint32_t* w, int64_t nw, int k, bool inverse, int64_t tid, int64_t nth, Sync sync) {
+#define CBT_CASE(K)                          \
+  case K:                                    \
+    if (!inverse) {                          \
+      cbt_hi_fix<K>(w, nw, tid, nth);        \
+      sync();                                \
+      cbt_lo_update<K>(w, nw, tid, nth);     \
+    } else {                                 \
+      cbt_lo_update<K>(w, nw, tid, nth);     \
+      sync();                                \
+      cbt_hi_fix<K>(w, nw, tid, nth);        \
+    }                                        \
+    break;
+  switch (k) {
+    CBT_CASE(5)
+    CBT_CASE(6)
+    CBT_CASE(7)
+    CBT_CASE(8)
+    CBT_CASE(9)
+    CBT_CASE(10)
+    CBT_CASE(11)
+    CBT_CASE(12)
+    CBT_CASE(13)
+    CBT_CASE(14)
+    CBT_CASE(15)
+    CBT_CASE(16)
+    CBT_CASE(17)
+    CBT_CASE(18)
+    CBT_CASE(19)
+    default:
+      break;
+  }
+#undef CBT_CASE
+  sync();
+}
+
+// whole conversion of a 2^mp-coefficient string (5 <= mp <= 20)
+template <typename Sync>
+CB_HD void cbt_convert(uint32_t* w, int mp, bool inverse, int64_t tid, int64_t nth, Sync sync) {
+  const int64_t nw = ((int64_t)1 << mp) >> 5;
+  if (!inverse) {
+    for (int k = mp - 1; k >= 5; k--) cbt_level(w, nw, k, false, tid, nth, sync);
+    cbt_word_levels(w, nw, false, tid, nth);
+    sync();
+  } else {
+    cbt_word_levels(w, nw, true, tid, nth);
+    sync();
+    for (int k = 5; k <= mp - 1; k++) cbt_level(w, nw, k, true, tid, nth, sync);
+  }
+}

@@ -71,6 +71,15 @@ static inline int gf32_cantor_basis(uint32_t beta[32]) {
 // turns into register renaming, plus the reduction t^32 = t^7 + t^3 + t^2 + 1
 // of the plane that wraps).  UNIFORM: c is the same in every lane of the wave
 // (branches on its bits are scalar); otherwise per-lane bit masks.
+// a ^ (b & c) as one gfx950 three-input bit operation (truth table 0x78 with
+// index src0*4 + src1*2 + src2); written as asm so that the compiler keeps
+// one instruction per term instead of re-associating the xor chain.
+__device__ __forceinline__ uint32_t gf32_xor_and(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x78" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
 template <bool UNIFORM>
 __device__ __forceinline__ void gf32_mulc_acc(const uint32_t* __restrict__ x, uint32_t c, uint32_t* __restrict__ y) {
   uint32_t w[32];
@@ -86,7 +95,7 @@ __device__ __forceinline__ void gf32_mulc_acc(const uint32_t* __restrict__ x, ui
     } else {
       const uint32_t m = 0u - ((c >> i) & 1u);
 #pragma unroll
-      for (int j = 0; j < 32; j++) y[j] ^= w[(j - i) & 31] & m;
+      for (int j = 0; j < 32; j++) y[j] = gf32_xor_and(y[j], w[(j - i) & 31], m);
     }
     // w <- w * t: physical slot (31 - i) & 31 holds the plane that wraps to
     // t^0; fold it into t^2, t^3, t^7 (slots relative to the rotation).

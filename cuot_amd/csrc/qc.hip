@@ -28,7 +28,7 @@
 //     and the top inverse level fused into the overlap-add / fold / xor x_0.
 #include <string.h>
 
-#include "conv_bits.h"
+#include "conv_fast.h"
 #include "gf32.h"
 #include "kernels.h"
 
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(1024) void qc_conv_rows(uint32_t* __restrict__ Bw, 
       for (int64_t i = threadIdx.x; i < tile_words; i += blockDim.x) sw[i] = g[i];
     }
     __syncthreads();
-    cb_convert(sw, mp, inverse != 0, threadIdx.x, blockDim.x, BarrierSync());
+    cbt_convert(sw, mp, inverse != 0, threadIdx.x, blockDim.x, BarrierSync());
     if ((tile_words & 3) == 0) {
       const int n4 = (int)(tile_words >> 2);
       uint4* g4 = (uint4*)g;
@@ -290,7 +290,7 @@ struct FftPassArgs {
   int64_t XS_words;  // instance stride of X in 32-bit words (blockIdx.y)
 };
 
-__device__ __attribute__((noinline)) void fft_layer(int lo, int R, int C, int lgC, uint32_t* lds, int64_t hib,
+__device__ __forceinline__ void fft_layer(int lo, int R, int C, int lgC, uint32_t* lds, int64_t hib,
                                                   int64_t cb, int k, bool inv) {
   const uint32_t* ctab = lds + (C << R) * FFT_STRIDE;
   const int s = k - lo;
@@ -338,7 +338,7 @@ __device__ __attribute__((noinline)) void fft_layer(int lo, int R, int C, int lg
 }
 
 // pointwise product by A^ (per-element constant) on the tile
-__device__ __attribute__((noinline)) void fft_pointwise(int lo, int R, int C, int lgC,
+__device__ __forceinline__ void fft_pointwise(int lo, int R, int C, int lgC,
                                                       const uint32_t* __restrict__ ahat, uint32_t* lds, int64_t hib,
                                                       int64_t cb) {
   const int TC = (1 << R) * C;
@@ -400,12 +400,20 @@ __global__ __launch_bounds__(FFT_THREADS) void qc_fft_pass(FftPassArgs a) {
       }
     }
     __syncthreads();
-    if (a.mode == PASS_FWD || a.mode == PASS_FUSED) {
-      for (int k = a.khi; k >= a.lo; k--) fft_layer(a.lo, a.R, a.C, a.lgC, lds, hib, cb, k, false);
-    }
-    if (a.mode == PASS_FUSED) fft_pointwise(a.lo, a.R, a.C, a.lgC, a.ahat, lds, hib, cb);
-    if (a.mode == PASS_INV || a.mode == PASS_FUSED) {
-      for (int k = a.lo; k <= a.khi; k++) fft_layer(a.lo, a.R, a.C, a.lgC, lds, hib, cb, k, true);
+    // one step loop so that the layer body is instantiated once: forward
+    // layers khi..lo, the pointwise product (fused pass), inverse layers lo..khi
+    const int nl = a.khi - a.lo + 1;
+    const int s_fwd = (a.mode == PASS_FWD || a.mode == PASS_FUSED) ? nl : 0;
+    const int s_pw = a.mode == PASS_FUSED ? 1 : 0;
+    const int s_inv = (a.mode == PASS_INV || a.mode == PASS_FUSED) ? nl : 0;
+    for (int step = 0; step < s_fwd + s_pw + s_inv; step++) {
+      if (step < s_fwd) {
+        fft_layer(a.lo, a.R, a.C, a.lgC, lds, hib, cb, a.khi - step, false);
+      } else if (step < s_fwd + s_pw) {
+        fft_pointwise(a.lo, a.R, a.C, a.lgC, a.ahat, lds, hib, cb);
+      } else {
+        fft_layer(a.lo, a.R, a.C, a.lgC, lds, hib, cb, a.lo + (step - s_fwd - s_pw), true);
+      }
     }
     uint4* Y4 = (uint4*)a.X;
     for (int f0 = threadIdx.x; f0 < n4; f0 += blockDim.x * FFT_LD) {
@@ -753,7 +761,7 @@ hipError_t fft_forward(const Bufs& b, int m, const uint32_t* ahat1, const uint32
   int khi = m - 1;
   bool first = true;
   const int RMAX = cuot_knobs().qc_radix > 0 ? cuot_knobs().qc_radix : 5;
-  const int CMAX = cuot_knobs().qc_cols > 0 ? cuot_knobs().qc_cols : 8;
+  const int CMAX = cuot_knobs().qc_cols > 0 ? cuot_knobs().qc_cols : FFT_TC;
   while (khi >= R0) {
     int R = khi - R0 + 1 < RMAX ? khi - R0 + 1 : RMAX;
     int lo = khi - R + 1;
@@ -773,7 +781,7 @@ hipError_t fft_inverse_upper(const Bufs& b, int m, int from, hipStream_t st) {
   const int64_t M = (int64_t)1 << m;
   int klo = from;
   const int RMAX = cuot_knobs().qc_radix > 0 ? cuot_knobs().qc_radix : 5;
-  const int CMAX = cuot_knobs().qc_cols > 0 ? cuot_knobs().qc_cols : 8;
+  const int CMAX = cuot_knobs().qc_cols > 0 ? cuot_knobs().qc_cols : FFT_TC;
   while (klo <= m - 1) {
     int R = m - klo < RMAX ? m - klo : RMAX;
     int C = CMAX;

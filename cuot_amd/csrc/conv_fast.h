@@ -7,6 +7,10 @@
 
 #include "conv_bits.h"
 
+struct alignas(16) cbt_w4 {
+  uint32_t x, y, z, w;
+};
+
 template <int J>
 CB_HD uint32_t cbt_lo_term(const uint32_t* hi, int64_t wj) {
   if constexpr (J >= 5) {
@@ -143,34 +147,109 @@ CB_HD uint32_t cbt_word_level(uint32_t x) {
 }
 
 // in-word levels 4..0 (forward) or 0..4 (inverse) of every word; needs mp >= 5
+CB_HD uint32_t cbt_word_all(uint32_t x, bool inverse) {
+  if (!inverse) {
+    x = cbt_word_level<4, false>(x);
+    x = cbt_word_level<3, false>(x);
+    x = cbt_word_level<2, false>(x);
+    x = cbt_word_level<1, false>(x);
+  } else {
+    x = cbt_word_level<1, true>(x);
+    x = cbt_word_level<2, true>(x);
+    x = cbt_word_level<3, true>(x);
+    x = cbt_word_level<4, true>(x);
+  }
+  return x;
+}
+
 CB_HD void cbt_word_levels(uint32_t* w, int64_t nw, bool inverse, int64_t tid, int64_t nth) {
-  for (int64_t i = tid; i < nw; i += nth) {
-    uint32_t x = w[i];
-    if (!inverse) {
-      x = cbt_word_level<4, false>(x);
-      x = cbt_word_level<3, false>(x);
-      x = cbt_word_level<2, false>(x);
-      x = cbt_word_level<1, false>(x);
-    } else {
-      x = cbt_word_level<1, true>(x);
-      x = cbt_word_level<2, true>(x);
-      x = cbt_word_level<3, true>(x);
-      x = cbt_word_level<4, true>(x);
+  if ((nw & 3) == 0) {
+    const int n4 = (int)(nw >> 2);
+    for (int i = (int)tid; i < n4; i += (int)nth) {
+      cbt_w4 v = ((const cbt_w4*)w)[i];
+      v.x = cbt_word_all(v.x, inverse);
+      v.y = cbt_word_all(v.y, inverse);
+      v.z = cbt_word_all(v.z, inverse);
+      v.w = cbt_word_all(v.w, inverse);
+      ((cbt_w4*)w)[i] = v;
     }
-    w[i] = x;
+  } else {
+    for (int64_t i = tid; i < nw; i += nth) w[i] = cbt_word_all(w[i], inverse);
+  }
+}
+
+// ---- four-word form of lo_update for levels with >= 4 words per half --------
+CB_HD void cbt_x4(cbt_w4& a, uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3) {
+  a.x ^= b0;
+  a.y ^= b1;
+  a.z ^= b2;
+  a.w ^= b3;
+}
+
+// L ^= hi[wj - e .. wj - e + 3] for e = 2^J bits; H = hi[wj .. wj + 3],
+// P = hi[wj - 4 .. wj - 1] (zero below the half)
+template <int J>
+CB_HD void cbt_lo_term4(cbt_w4& L, const cbt_w4& H, const cbt_w4& P, const uint32_t* hi, int wj) {
+  if constexpr (J >= 7) {
+    constexpr int qw = 1 << (J - 5);
+    if (wj >= qw) {
+      const cbt_w4 Q = *(const cbt_w4*)(hi + wj - qw);
+      cbt_x4(L, Q.x, Q.y, Q.z, Q.w);
+    }
+  } else if constexpr (J == 6) {
+    cbt_x4(L, P.z, P.w, H.x, H.y);
+  } else if constexpr (J == 5) {
+    cbt_x4(L, P.w, H.x, H.y, H.z);
+  } else {
+    constexpr int r = 1 << J;
+    cbt_x4(L, (H.x << r) | (P.w >> (32 - r)), (H.y << r) | (H.x >> (32 - r)), (H.z << r) | (H.y >> (32 - r)),
+           (H.w << r) | (H.z >> (32 - r)));
+  }
+}
+
+template <int K, int J = 0>
+CB_HD void cbt_lo_terms4(cbt_w4& L, const cbt_w4& H, const cbt_w4& P, const uint32_t* hi, int wj) {
+  if constexpr (J < K) {
+    if constexpr ((J & ~K) == 0) cbt_lo_term4<J>(L, H, P, hi, wj);
+    cbt_lo_terms4<K, J + 1>(L, H, P, hi, wj);
+  }
+}
+
+template <int K>
+CB_HD void cbt_lo_update4(uint32_t* w, int nw, int tid, int nth) {
+  static_assert(K >= 7, "four-word form needs >= 4 words per half");
+  constexpr int lhw = K - 5;
+  constexpr int hw = 1 << lhw;
+  const int total4 = nw >> 3;  // groups of four lo words
+  for (int t = tid; t < total4; t += nth) {
+    const int t4 = t << 2;
+    const int g = t4 >> lhw, wj = t4 & (hw - 1);
+    uint32_t* lo = w + (g << (lhw + 1));
+    const uint32_t* hi = lo + hw;
+    cbt_w4 L = *(const cbt_w4*)(lo + wj);
+    const cbt_w4 H = *(const cbt_w4*)(hi + wj);
+    cbt_w4 P = {0u, 0u, 0u, 0u};
+    if (wj >= 4) P = *(const cbt_w4*)(hi + wj - 4);
+    cbt_lo_terms4<K>(L, H, P, hi, wj);
+    *(cbt_w4*)(lo + wj) = L;
   }
 }
 
 template <typename Sync>
 CB_HD void cbt_level(uint32_t* w, int64_t nw, int k, bool inverse, int64_t tid, int64_t nth, Sync sync) {
+#define CBT_LO(K)                                                      \
+  if constexpr (K >= 7)                                                \
+    cbt_lo_update4<K>(w, (int)nw, (int)tid, (int)nth);                 \
+  else                                                                 \
+    cbt_lo_update<K>(w, nw, tid, nth);
 #define CBT_CASE(K)                          \
   case K:                                    \
     if (!inverse) {                          \
       cbt_hi_fix<K>(w, nw, tid, nth);        \
       sync();                                \
-      cbt_lo_update<K>(w, nw, tid, nth);     \
+      CBT_LO(K)                              \
     } else {                                 \
-      cbt_lo_update<K>(w, nw, tid, nth);     \
+      CBT_LO(K)                              \
       sync();                                \
       cbt_hi_fix<K>(w, nw, tid, nth);        \
     }                                        \
@@ -195,6 +274,7 @@ CB_HD void cbt_level(uint32_t* w, int64_t nw, int k, bool inverse, int64_t tid, 
       break;
   }
 #undef CBT_CASE
+#undef CBT_LO
   sync();
 }
 

@@ -36,10 +36,12 @@ namespace {
 
 constexpr int CONV_B = 12;     // conversion levels done in LDS (tile 2^B elements of one plane)
 constexpr int CONV_G = 4;      // planes per conversion group (ping-pong buffer size)
-constexpr int FFT_TC = 256;    // elements per FFT LDS tile (2^R * C)
+constexpr int FFT_TC = 128;    // elements per FFT LDS tile (2^R * C): two tiles fit one CU's LDS
+constexpr int FFT_LOGTC = 7;
 constexpr int FFT_STRIDE = 132;  // LDS words per element (32 planes x 4 words + pad)
-constexpr int FFT_THREADS = 512;
+constexpr int FFT_THREADS = 256;  // one (butterfly, word) task per lane per layer
 constexpr int FFT_LD = 16;  // 16-byte loads in flight per lane (tile staging)
+constexpr int FFT_WGS = 512;  // FFT pass grid: two resident WGs on each of the 256 CUs
 constexpr int CR_LD = 8;    // same for the bit-row conversion tiles
 // the row-layout conversion handles strings of 2^5 .. 2^20 coefficients (128 KB LDS)
 constexpr int ROW_MP_MIN = 5, ROW_MP_MAX = 20;
@@ -306,32 +308,36 @@ __device__ __forceinline__ void fft_layer(int lo, int R, int C, int lgC, uint32_
     const uint64_t g = (uint64_t)idx >> (k + 1);
     uint32_t* pu = lds + (tu * C + c) * FFT_STRIDE + w;
     uint32_t* pv = lds + (tv * C + c) * FFT_STRIDE + w;
-    uint32_t xu[32], xv[32];
-#pragma unroll
-    for (int p = 0; p < 32; p++) {
-      xu[p] = pu[4 * p];
-      xv[p] = pv[4 * p];
-    }
-    const uint32_t g0 = __builtin_amdgcn_readfirstlane((uint32_t)g);
-    const bool uni = __all((uint32_t)g == g0);
+    // forward: u += c*v, v += u; inverse: v += u, u += c*v.  Only v (or the
+    // new v) and the product are held in registers; u streams through LDS.
+    uint32_t xv[32], acc[32];
     if (inv) {
 #pragma unroll
-      for (int p = 0; p < 32; p++) xv[p] ^= xu[p];
-    }
-    if (uni) {
-      const uint32_t cst = __builtin_amdgcn_readfirstlane(ctab_const(ctab, g0));
-      gf32_mulc_acc<true>(xv, cst, xu);
+      for (int p = 0; p < 32; p++) {
+        xv[p] = pv[4 * p] ^ pu[4 * p];
+        pv[4 * p] = xv[p];
+      }
     } else {
-      gf32_mulc_acc<false>(xv, ctab_const(ctab, (uint32_t)g), xu);
-    }
-    if (!inv) {
 #pragma unroll
-      for (int p = 0; p < 32; p++) xv[p] ^= xu[p];
+      for (int p = 0; p < 32; p++) xv[p] = pv[4 * p];
     }
+    const uint32_t g0 = __builtin_amdgcn_readfirstlane((uint32_t)g);
+    if (__all((uint32_t)g == g0)) {
+      const uint32_t cst = __builtin_amdgcn_readfirstlane(ctab_const(ctab, g0));
+      gf32_mulc_top<true>(xv, cst, acc);
+    } else {
+      gf32_mulc_top<false>(xv, ctab_const(ctab, (uint32_t)g), acc);
+    }
+    if (inv) {
 #pragma unroll
-    for (int p = 0; p < 32; p++) {
-      pu[4 * p] = xu[p];
-      pv[4 * p] = xv[p];
+      for (int p = 0; p < 32; p++) pu[4 * p] ^= acc[p];
+    } else {
+#pragma unroll
+      for (int p = 0; p < 32; p++) {
+        const uint32_t un = pu[4 * p] ^ acc[p];
+        pu[4 * p] = un;
+        pv[4 * p] = xv[p] ^ un;
+      }
     }
   }
   __syncthreads();
@@ -351,14 +357,14 @@ __device__ __forceinline__ void fft_pointwise(int lo, int R, int C, int lgC,
     uint32_t x[32], y[32];
 #pragma unroll
     for (int p = 0; p < 32; p++) x[p] = pe[4 * p];
-    gf32_mulc<false>(x, ahat[idx], y);
+    gf32_mulc_top<false>(x, ahat[idx], y);
 #pragma unroll
     for (int p = 0; p < 32; p++) pe[4 * p] = y[p];
   }
   __syncthreads();
 }
 
-__global__ __launch_bounds__(FFT_THREADS) void qc_fft_pass(FftPassArgs a) {
+__global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void qc_fft_pass(FftPassArgs a) {
   extern __shared__ uint32_t lds[];
   a.X += (int64_t)blockIdx.y * a.XS_words;
   if (a.ahats) a.ahat = a.ahats[blockIdx.y];
@@ -368,35 +374,38 @@ __global__ __launch_bounds__(FFT_THREADS) void qc_fft_pass(FftPassArgs a) {
   const int64_t ntiles = a.M / TC;
   const int64_t half = a.M >> 1;
   ctab_fill(lds + TC * FFT_STRIDE, threadIdx.x, blockDim.x);  // read after the first barrier
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int64_t cb = tile & (cblocks - 1), hib = tile >> (a.lo - a.lgC);
-    // load: one 16-byte plane word group per lane, f = (p*T + t)*C + c
-    // (c, t fastest: contiguous runs); FFT_LD loads in flight per lane
-    const int n4 = TC * 32;
-    const uint4* X4 = (const uint4*)a.X;
-    for (int f0 = threadIdx.x; f0 < n4; f0 += blockDim.x * FFT_LD) {
-      uint4 v[FFT_LD];
+  // Each lane stages up to FFT_LD 16-byte plane words of a tile (f = (p*T +
+  // t)*C + c: c, t fastest, contiguous runs; TC * 32 <= FFT_LD * blockDim).
+  // Two WGs share a CU, so one's loads and stores overlap the other's layers.
+  const int n4 = TC * 32;
+  const uint4* X4 = (const uint4*)a.X;
+  uint4 v[FFT_LD];
+  auto load_tile = [&](int64_t tl) {
+    const int64_t lcb = tl & (cblocks - 1), lhib = tl >> (a.lo - a.lgC);
 #pragma unroll
-      for (int u = 0; u < FFT_LD; u++) {
-        const int f = f0 + u * (int)blockDim.x;
-        v[u] = z4();
-        if (f < n4) {
-          const int c = f & (a.C - 1);
-          const int pt = f >> a.lgC;
-          const int t = pt & (T - 1), p = pt >> a.R;
-          const int64_t idx = (hib << (a.lo + a.R)) | ((int64_t)t << a.lo) | (cb * a.C + c);
-          if (!a.first || (idx < half && p < 16)) v[u] = X4[(int64_t)p * a.M + idx];
-        }
+    for (int u = 0; u < FFT_LD; u++) {
+      const int f = (int)threadIdx.x + u * (int)blockDim.x;
+      v[u] = z4();
+      if (f < n4) {
+        const int c = f & (a.C - 1);
+        const int pt = f >> a.lgC;
+        const int t = pt & (T - 1), p = pt >> a.R;
+        const int64_t idx = (lhib << (a.lo + a.R)) | ((int64_t)t << a.lo) | (lcb * a.C + c);
+        if (!a.first || (idx < half && p < 16)) v[u] = X4[(int64_t)p * a.M + idx];
       }
+    }
+  };
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    load_tile(tile);
+    const int64_t cb = tile & (cblocks - 1), hib = tile >> (a.lo - a.lgC);
 #pragma unroll
-      for (int u = 0; u < FFT_LD; u++) {
-        const int f = f0 + u * (int)blockDim.x;
-        if (f < n4) {
-          const int c = f & (a.C - 1);
-          const int pt = f >> a.lgC;
-          const int t = pt & (T - 1), p = pt >> a.R;
-          *(uint4*)&lds[(t * a.C + c) * FFT_STRIDE + p * 4] = v[u];
-        }
+    for (int u = 0; u < FFT_LD; u++) {
+      const int f = (int)threadIdx.x + u * (int)blockDim.x;
+      if (f < n4) {
+        const int c = f & (a.C - 1);
+        const int pt = f >> a.lgC;
+        const int t = pt & (T - 1), p = pt >> a.R;
+        *(uint4*)&lds[(t * a.C + c) * FFT_STRIDE + p * 4] = v[u];
       }
     }
     __syncthreads();
@@ -407,26 +416,24 @@ __global__ __launch_bounds__(FFT_THREADS) void qc_fft_pass(FftPassArgs a) {
     const int s_pw = a.mode == PASS_FUSED ? 1 : 0;
     const int s_inv = (a.mode == PASS_INV || a.mode == PASS_FUSED) ? nl : 0;
     for (int step = 0; step < s_fwd + s_pw + s_inv; step++) {
-      if (step < s_fwd) {
-        fft_layer(a.lo, a.R, a.C, a.lgC, lds, hib, cb, a.khi - step, false);
-      } else if (step < s_fwd + s_pw) {
+      if (step == s_fwd && s_pw) {
         fft_pointwise(a.lo, a.R, a.C, a.lgC, a.ahat, lds, hib, cb);
       } else {
-        fft_layer(a.lo, a.R, a.C, a.lgC, lds, hib, cb, a.lo + (step - s_fwd - s_pw), true);
+        const bool inv = step >= s_fwd;
+        const int k = inv ? a.lo + (step - s_fwd - s_pw) : a.khi - step;
+        fft_layer(a.lo, a.R, a.C, a.lgC, lds, hib, cb, k, inv);
       }
     }
     uint4* Y4 = (uint4*)a.X;
-    for (int f0 = threadIdx.x; f0 < n4; f0 += blockDim.x * FFT_LD) {
 #pragma unroll
-      for (int u = 0; u < FFT_LD; u++) {
-        const int f = f0 + u * (int)blockDim.x;
-        if (f < n4) {
-          const int c = f & (a.C - 1);
-          const int pt = f >> a.lgC;
-          const int t = pt & (T - 1), p = pt >> a.R;
-          const int64_t idx = (hib << (a.lo + a.R)) | ((int64_t)t << a.lo) | (cb * a.C + c);
-          Y4[(int64_t)p * a.M + idx] = *(const uint4*)&lds[(t * a.C + c) * FFT_STRIDE + p * 4];
-        }
+    for (int u = 0; u < FFT_LD; u++) {
+      const int f = (int)threadIdx.x + u * (int)blockDim.x;
+      if (f < n4) {
+        const int c = f & (a.C - 1);
+        const int pt = f >> a.lgC;
+        const int t = pt & (T - 1), p = pt >> a.R;
+        const int64_t idx = (hib << (a.lo + a.R)) | ((int64_t)t << a.lo) | (cb * a.C + c);
+        Y4[(int64_t)p * a.M + idx] = *(const uint4*)&lds[(t * a.C + c) * FFT_STRIDE + p * 4];
       }
     }
     __syncthreads();
@@ -718,7 +725,12 @@ hipError_t conv_inverse(Bufs b, int64_t M, int P, int mtop, hipStream_t st) {
   return cuot_launch_transpose_planes2(b.Bw, M, b.X, true, b.B * P, P, inst_words, 128 * row_words, b.XS, M, st);
 }
 
-void pass_grid(int64_t ntiles, unsigned& g) { g = (unsigned)(ntiles < 4096 ? ntiles : 4096); }
+// two resident WGs per CU; every WG walks many tiles
+void pass_grid(int64_t ntiles, int B, unsigned& g) {
+  int64_t cap = FFT_WGS / (B < 1 ? 1 : B);
+  if (cap < 1) cap = 1;
+  g = (unsigned)(ntiles < cap ? ntiles : cap);
+}
 
 FftPassArgs pass_args(const Bufs& b, int64_t M, int lo, int R, int C, int mode, int khi, int first,
                       const uint32_t* ahat1, const uint32_t* const* ahats) {
@@ -740,7 +752,7 @@ FftPassArgs pass_args(const Bufs& b, int64_t M, int lo, int R, int C, int mode, 
 
 void launch_pass(const FftPassArgs& a, int B, hipStream_t st) {
   unsigned g;
-  pass_grid(a.M / ((1 << a.R) * a.C), g);
+  pass_grid(a.M / ((1 << a.R) * a.C), B, g);
   // algorithmic bytes: every element's 32 planes read and written once (the
   // first pass reads only planes 0..15 of the lower half)
   const double per = (double)a.M * 512.0 * B;
@@ -757,7 +769,7 @@ void launch_pass(const FftPassArgs& a, int B, hipStream_t st) {
 hipError_t fft_forward(const Bufs& b, int m, const uint32_t* ahat1, const uint32_t* const* ahats, bool fused,
                        int* low_khi, hipStream_t st) {
   const int64_t M = (int64_t)1 << m;
-  const int R0 = m < 8 ? m : 8;
+  const int R0 = m < FFT_LOGTC ? m : FFT_LOGTC;
   int khi = m - 1;
   bool first = true;
   const int RMAX = cuot_knobs().qc_radix > 0 ? cuot_knobs().qc_radix : 5;

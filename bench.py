@@ -143,6 +143,26 @@ def batch_leg(torch, cuot_amd, ctx, steps):
 
     plain_ms = timed(run_plain)
     batch_ms = timed(run_batch)
+    # where the plain form spends its time: per-instance events, by kind and size
+    evs = []
+    for item in plain:
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        wi, cfg, ws, o0, o1, b, d = item
+        e0.record(stream)
+        if wi.kind == SOT:
+            ctx.sot_run(cfg, ws, o0, o1, b, d, stream)
+        else:
+            ctx.ferret_run(cfg, ws, o0, o1, d, stream)
+        e1.record(stream)
+        evs.append((wi, e0, e1))
+    torch.cuda.synchronize()
+    by_size = {}
+    for wi, e0, e1 in evs:
+        key = f"{'sot' if wi.kind == SOT else 'ferret'}_log2n_{int(wi.n).bit_length() - 1}"
+        ent = by_size.setdefault(key, {"instances": 0, "cots": 0, "ms": 0.0})
+        ent["instances"] += 1
+        ent["cots"] += wi.n
+        ent["ms"] = round(ent["ms"] + e0.elapsed_time(e1), 4)
     exact = True
     for i, (wi, cfg, ws, o0, o1, b, d) in enumerate(plain):
         exact &= bool(torch.equal(o0, bt.out0[i])) and bool(torch.equal(o1, bt.out1[i]))
@@ -156,6 +176,7 @@ def batch_leg(torch, cuot_amd, ctx, steps):
         "batched": {"value": round(total / (batch_ms / 1e3), 1), "unit": "COTs/s", "ms": round(batch_ms, 3),
                     "form": "one launch sequence per stage over all instances, HIP graph replay"},
         "speedup": round(plain_ms / batch_ms, 3),
+        "plain_by_size": dict(sorted(by_size.items())),
         "byte_exact_vs_plain": exact,
     }
     bt.close()

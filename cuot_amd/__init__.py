@@ -212,11 +212,12 @@ class Server:
     """The serving loop (cuot_server_*): submit requests, poll results, release."""
 
     def __init__(self, ctx: Context, hbm_budget: int, max_batch: int = 0, streams: int = 0, qc_chunk_bytes: int = 0,
-                 use_graph: bool = False, plain: bool = False):
+                 use_graph: bool = False, plain: bool = False, max_batch_cots: int = 0):
         self.ctx = ctx
         self.lib = ctx.lib
         self.cfg = ServerConfig(hbm_budget=hbm_budget, max_batch=max_batch, streams=streams,
-                                qc_chunk_bytes=qc_chunk_bytes, use_graph=int(use_graph), plain=int(plain))
+                                qc_chunk_bytes=qc_chunk_bytes, use_graph=int(use_graph), plain=int(plain),
+                                max_batch_cots=max_batch_cots)
         h = C.c_void_p()
         check(self.lib.cuot_server_new(ctx.handle, C.byref(self.cfg), C.byref(h)), "cuot_server_new")
         self.handle = h
@@ -253,13 +254,13 @@ class Server:
 
 
 def serve_replay(ctx: Context, instances, arrivals, hbm_budget: int, plain: bool = False, max_batch: int = 0,
-                 streams: int = 0, qc_chunk_bytes: int = 0, use_graph: bool = False):
+                 streams: int = 0, qc_chunk_bytes: int = 0, use_graph: bool = False, max_batch_cots: int = 0):
     """Real-time replay of (arrival seconds, Instance) through a fresh server.
 
     Returns (latencies in s as numpy array, wall seconds, batches launched)."""
     n = len(instances)
     cfg = ServerConfig(hbm_budget=hbm_budget, max_batch=max_batch, streams=streams, qc_chunk_bytes=qc_chunk_bytes,
-                       use_graph=int(use_graph), plain=int(plain))
+                       use_graph=int(use_graph), plain=int(plain), max_batch_cots=max_batch_cots)
     arr = (Instance * n)(*instances)
     t = (C.c_double * n)(*[float(a) for a in arrivals])
     lat = (C.c_double * n)()

@@ -198,9 +198,12 @@ int cuot_server::launch_batch(std::unique_lock<std::mutex>& lk, int s) {
   cuot_batch* b = nullptr;
   size_t ws = 0, outs = 0;
   const int maxb = cfg.plain ? 1 : (cfg.max_batch > 0 ? cfg.max_batch : 64);
+  const int64_t max_cots = cfg.max_batch_cots > 0 ? cfg.max_batch_cots : ((int64_t)1 << 24);
+  int64_t cots = 0;
   for (int64_t t : queue) {
     if ((int)take.size() >= maxb) break;
     Req& r = reqs[t];
+    if (!take.empty() && cots + r.inst.n > max_cots) break;  // keep batches short
     std::vector<cuot_instance> trial = insts;
     trial.push_back(r.inst);
     size_t tws = 0;
@@ -243,6 +246,7 @@ int cuot_server::launch_batch(std::unique_lock<std::mutex>& lk, int s) {
     b = tb;
     ws = tws;
     outs = tout;
+    cots += r.inst.n;
     take.push_back(t);
     insts.swap(trial);
   }

@@ -328,6 +328,7 @@ typedef struct {
   size_t qc_chunk_bytes; /* cuot_batch_new knob (0 = default) */
   int32_t use_graph;     /* replay batches through captured graphs */
   int32_t plain;         /* FIFO plain comparator */
+  int64_t max_batch_cots; /* cap on the COTs of one batch (0 = 2^24); bounds batch duration */
 } cuot_server_config;
 
 typedef struct {

@@ -125,6 +125,11 @@ SIGNATURES = {
     "cuot_get_knobs": (INT, [C.POINTER(KnobValues)]),
     "cuot_timing_begin": (INT, [P]),
     "cuot_timing_end": (INT, [P, C.POINTER(LaunchTime), INT, C.POINTER(INT)]),
+    "cuot_ferret_session_new": (INT, [P, C.POINTER(FerretConfig), C.POINTER(P)]),
+    "cuot_ferret_session_info": (INT, [P, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64)]),
+    "cuot_ferret_session_delta": (INT, [P, P, P]),
+    "cuot_ferret_session_extend": (INT, [P, P, P, P]),
+    "cuot_ferret_session_free": (INT, [P]),
     "cuot_serve_replay": (INT, [P, C.POINTER(ServerConfig), C.POINTER(Instance), C.POINTER(C.c_double), INT,
                                 C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(I64)]),
 }

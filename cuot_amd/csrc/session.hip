@@ -1,0 +1,260 @@
+// session.hip -- an iterating, device-resident Ferret session (emp-ot's
+// FerretCOT extend loop, ferret_cot.hpp:79-103, 171-209), both roles on one GPU.
+//
+// The session keeps Delta and a bootstrap of M = k + h*t + 128 COTs per party
+// in HBM.  One extend:
+//   1. pre-OT (preot.h:38-54): the first h*t bootstrap COTs become random OTs
+//      through the circular correlation-robust hash H: sender pads
+//      P0 = H(K), P1 = H(K ^ Delta); receiver bits = LSB(M), Pr = H(M);
+//   2. MPCOT: sender GGM trees (roots from the seed, stream 1, advanced by t
+//      per extend), level sums sent as m0 ^ P0 / m1 ^ P1; the receiver's
+//      choice per level is its bit, its chosen sum pad_b ^ Pr (its own view);
+//      the secret sum closes the punctured leaf (spcot_recver.h:53-64);
+//   3. LPN with kk = bootstrap + 128 (ferret_cot.hpp:89);
+//   4. the last M outputs become the next bootstrap (ferret_cot.hpp:100).
+// H(x) = AES0(s(x)) ^ s(x) with s(x) = (hi ^ lo, hi) as (hi, lo) halves and
+// AES0 = AES-128 under the all-zero key.
+#include <string.h>
+
+#include "aes_dev.h"
+#include "ctx.h"
+#include "engine.h"
+#include "kernels.h"
+
+namespace {
+
+struct CcrhArgs {
+  const uint32_t* T0;
+  uint32_t rk[44];
+  const blk* boot;
+  int64_t npre;
+  const blk* delta;
+  int recv;
+  blk* out0;     // P0 (sender) or Pr (receiver)
+  blk* out1;     // P1 (sender)
+  uint8_t* bits; // receiver choice bits
+};
+
+CUOT_DEV blk ccrh(const AesLds& aes, const uint32_t* rk, blk x) {
+  // s(x): hi' = hi ^ lo, lo' = hi (hi = words 2,3)
+  blk s;
+  s.x = x.z;
+  s.y = x.w;
+  s.z = x.z ^ x.x;
+  s.w = x.w ^ x.y;
+  return bxor(aes_enc(aes, rk, s), s);
+}
+
+__global__ __launch_bounds__(256) void ccrh_pre_kernel(CcrhArgs a) {
+  __shared__ uint32_t tab[CUOT_AES_LDS_WORDS];
+  aes_stage_table(tab, a.T0, threadIdx.x, 256);
+  __syncthreads();
+  const AesLds aes = aes_lds(tab);
+  const blk D = *a.delta;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.npre; i += (int64_t)gridDim.x * blockDim.x) {
+    const blk v = a.boot[i];
+    if (!a.recv) {
+      a.out0[i] = ccrh(aes, a.rk, v);
+      a.out1[i] = ccrh(aes, a.rk, bxor(v, D));
+    } else {
+      a.out0[i] = ccrh(aes, a.rk, v);
+      a.bits[i] = (uint8_t)(v.x & 1u);
+    }
+  }
+}
+
+// receiver's chosen level sums from the sender's pads (sums [tree][level][L,R])
+__global__ void ot_unmask_kernel(const blk* __restrict__ sums, const blk* __restrict__ P0, const blk* __restrict__ P1,
+                                 const blk* __restrict__ Pr, const uint8_t* __restrict__ bits, int64_t npre,
+                                 blk* __restrict__ csum) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npre; i += (int64_t)gridDim.x * blockDim.x) {
+    const blk pad0 = bxor(sums[2 * i], P0[i]);
+    const blk pad1 = bxor(sums[2 * i + 1], P1[i]);
+    csum[i] = bxor(bits[i] ? pad1 : pad0, Pr[i]);
+  }
+}
+
+unsigned grid_of(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g < 1) g = 1;
+  if (g > 4096) g = 4096;
+  return (unsigned)g;
+}
+
+}  // namespace
+
+struct cuot_ferret_session {
+  cuot_ctx* ctx = nullptr;
+  FerretPlan plan;  // parameters, GGM/LPN argument builders
+  int64_t npre = 0, M = 0, ot_limit = 0, round = 0;
+  char* mem = nullptr;
+  blk *delta, *boot_s, *boot_r, *P0, *P1, *Pr, *roots, *sums, *csum, *ssum, *cbits;
+  uint8_t* bits;
+};
+
+namespace {
+
+size_t carve_session(cuot_ferret_session* s, char* base) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    off = (off + 255) & ~(size_t)255;
+    char* p = base ? base + off : nullptr;
+    off += bytes;
+    return p;
+  };
+  const int64_t t = s->plan.cfg.t, h = s->plan.cfg.h;
+  s->delta = (blk*)take(16);
+  s->boot_s = (blk*)take((size_t)s->M * 16);
+  s->boot_r = (blk*)take((size_t)s->M * 16);
+  s->P0 = (blk*)take((size_t)s->npre * 16);
+  s->P1 = (blk*)take((size_t)s->npre * 16);
+  s->Pr = (blk*)take((size_t)s->npre * 16);
+  s->bits = (uint8_t*)take((size_t)s->npre);
+  s->roots = (blk*)take((size_t)t * 16);
+  s->sums = (blk*)take((size_t)2 * t * h * 16);
+  s->csum = (blk*)take((size_t)t * h * 16);
+  s->ssum = (blk*)take((size_t)t * 16);
+  s->cbits = (blk*)take((size_t)((s->M + 127) / 128) * 16);
+  return off + 256;
+}
+
+hipError_t extend(cuot_ferret_session* s, blk* out_s, blk* out_r, hipStream_t st) {
+  const cuot_ferret_config& cfg = s->plan.cfg;
+  hipError_t e;
+  // 1. pre-OT through the hash
+  CcrhArgs c;
+  memset(&c, 0, sizeof(c));
+  c.T0 = cuot_device_T0();
+  const cuot_block zero = {0, 0};
+  cuot_aes_expand_host((const uint8_t*)&zero, c.rk);
+  c.npre = s->npre;
+  c.delta = s->delta;
+  c.boot = s->boot_s;
+  c.out0 = s->P0;
+  c.out1 = s->P1;
+  hipLaunchKernelGGL(ccrh_pre_kernel, dim3(grid_of(s->npre)), dim3(256), 0, st, c);
+  c.recv = 1;
+  c.boot = s->boot_r;
+  c.out0 = s->Pr;
+  c.out1 = nullptr;
+  c.bits = s->bits;
+  hipLaunchKernelGGL(ccrh_pre_kernel, dim3(grid_of(s->npre)), dim3(256), 0, st, c);
+  if ((e = hipGetLastError())) return e;
+  // 2. sender MPCOT
+  uint32_t rk_seed[44];
+  cuot_aes_expand_host((const uint8_t*)&cfg.seed, rk_seed);
+  if ((e = cuot_launch_prg(rk_seed, 1, (uint64_t)(s->round * cfg.t), cfg.t, s->roots, st))) return e;
+  GgmKArgs a;
+  s->plan.ggm_args(a);
+  a.roots = s->roots;
+  a.leaves = out_s;
+  a.sums = s->sums;
+  if ((e = cuot_launch_ggm(a, false, st))) return e;
+  if ((e = cuot_launch_ferret_sums(s->sums, s->delta, nullptr, cfg.t, cfg.h, s->ssum, nullptr, nullptr, nullptr, st)))
+    return e;
+  hipLaunchKernelGGL(ot_unmask_kernel, dim3(grid_of(s->npre)), dim3(256), 0, st, s->sums, s->P0, s->P1, s->Pr, s->bits,
+                     s->npre, s->csum);
+  if ((e = hipGetLastError())) return e;
+  //    receiver MPCOT from its bits and unmasked sums
+  GgmKArgs r;
+  s->plan.ggm_args(r);
+  r.leaves = out_r;
+  r.choice = s->bits;
+  r.csum = s->csum;
+  r.fin = s->ssum;
+  if ((e = cuot_launch_ggm(r, true, st))) return e;
+  // 3. LPN with kk = bootstrap + 128
+  LpnLaunch l;
+  s->plan.lpn_launch(out_s, out_r, l);
+  l.kk[0] = s->boot_s + 128;
+  l.kk[1] = s->boot_r + 128;
+  if ((e = cuot_launch_lpn(l, st))) return e;
+  // 4. refresh the bootstrap from the last M outputs
+  if ((e = hipMemcpyAsync(s->boot_s, out_s + s->ot_limit, (size_t)s->M * 16, hipMemcpyDeviceToDevice, st))) return e;
+  if ((e = hipMemcpyAsync(s->boot_r, out_r + s->ot_limit, (size_t)s->M * 16, hipMemcpyDeviceToDevice, st))) return e;
+  s->round++;
+  return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" int cuot_ferret_session_new(cuot_ctx* ctx, const cuot_ferret_config* cfg, cuot_ferret_session** out) {
+  CUOT_CHECK_ARG(ctx && cfg && out, "NULL argument");
+  *out = nullptr;
+  cuot_ferret_session* s = new cuot_ferret_session();
+  s->ctx = ctx;
+  s->plan.cfg = *cfg;
+  if (s->plan.cfg.ntasks < 1) s->plan.cfg.ntasks = 1;
+  int rc = s->plan.validate();
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  s->npre = (int64_t)cfg->h * cfg->t;
+  s->M = cfg->k + s->npre + 128;
+  s->ot_limit = cfg->n - s->M;
+  if (s->ot_limit <= 0) {
+    delete s;
+    cuot_set_last_error("n must exceed k + h*t + 128 (the bootstrap carried to the next extend)", __FILE__,
+                        __LINE__);
+    return CUOT_ERR_ARG;
+  }
+  const size_t bytes = carve_session(s, nullptr);
+  if (hipSetDevice(ctx->device) != hipSuccess || hipMalloc((void**)&s->mem, bytes) != hipSuccess) {
+    delete s;
+    cuot_set_last_error("cannot allocate the session state", __FILE__, __LINE__);
+    return CUOT_ERR_NOMEM;
+  }
+  carve_session(s, s->mem);
+  // Delta and the initial bootstrap from the seed (streams 0, 7, 8)
+  uint32_t rk[44];
+  cuot_aes_expand_host((const uint8_t*)&cfg->seed, rk);
+  hipStream_t st = ctx->stream;
+  hipError_t e = cuot_launch_prg(rk, 0, 0, 1, s->delta, st);
+  if (!e) e = cuot_launch_setup_fix(s->delta, nullptr, 0, cfg->h, nullptr, nullptr, nullptr, 0, 0, st);
+  if (!e) e = cuot_launch_prg(rk, 7, 0, s->M, s->boot_s, st);
+  if (!e) e = cuot_launch_prg(rk, 8, 0, (s->M + 127) / 128, s->cbits, st);
+  if (!e) e = cuot_launch_ferret_boot(s->boot_s, s->boot_r, (const uint8_t*)s->cbits, s->delta, s->M, st);
+  if (!e) e = hipStreamSynchronize(st);
+  if (e) {
+    hipFree(s->mem);
+    delete s;
+    CUOT_HIP_TRY(e);
+  }
+  *out = s;
+  return CUOT_OK;
+}
+
+extern "C" int cuot_ferret_session_info(const cuot_ferret_session* s, int64_t* M, int64_t* ot_limit, int64_t* round) {
+  CUOT_CHECK_ARG(s, "NULL session");
+  if (M) *M = s->M;
+  if (ot_limit) *ot_limit = s->ot_limit;
+  if (round) *round = s->round;
+  return CUOT_OK;
+}
+
+extern "C" int cuot_ferret_session_delta(const cuot_ferret_session* s, cuot_block* delta_d, void* stream) {
+  CUOT_CHECK_ARG(s && delta_d, "NULL argument");
+  CUOT_HIP_TRY(hipSetDevice(s->ctx->device));
+  CUOT_HIP_TRY(hipMemcpyAsync(delta_d, s->delta, 16, hipMemcpyDeviceToDevice, s->ctx->st(stream)));
+  return CUOT_OK;
+}
+
+extern "C" int cuot_ferret_session_extend(cuot_ferret_session* s, cuot_block* out_s_d, cuot_block* out_r_d,
+                                          void* stream) {
+  CUOT_CHECK_ARG(s && out_s_d && out_r_d, "NULL argument");
+  CUOT_HIP_TRY(hipSetDevice(s->ctx->device));
+  CUOT_HIP_TRY(extend(s, (blk*)out_s_d, (blk*)out_r_d, s->ctx->st(stream)));
+  return CUOT_OK;
+}
+
+extern "C" int cuot_ferret_session_free(cuot_ferret_session* s) {
+  if (!s) return CUOT_OK;
+  if (s->mem) {
+    hipSetDevice(s->ctx->device);
+    hipDeviceSynchronize();
+    hipFree(s->mem);
+  }
+  delete s;
+  return CUOT_OK;
+}

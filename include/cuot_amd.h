@@ -398,6 +398,22 @@ typedef struct {
 int cuot_timing_begin(cuot_ctx *ctx);
 int cuot_timing_end(cuot_ctx *ctx, cuot_launch_time *out, int max_out, int *count);
 
+/* ---- iterating Ferret session ---------------------------------------------------
+ * Replaces FerretCOT's extend loop (ferret_cot.hpp:79-103 extend / extend_f2k,
+ * 171-209 rcot, preot.h:38-54 OTPre) with both parties on one GPU and the
+ * state in HBM: Delta and a bootstrap of M = k + h*t + 128 COTs per party.
+ * Each extend turns the first h*t bootstrap COTs into the MPCOT's base OTs
+ * through a correlation-robust hash, expands the trees, runs the LPN with
+ * kk = bootstrap + 128, writes n COTs per party (the first ot_limit = n - M
+ * are the caller's) and keeps the last M as the next bootstrap. */
+typedef struct cuot_ferret_session cuot_ferret_session;
+int cuot_ferret_session_new(cuot_ctx *ctx, const cuot_ferret_config *cfg, cuot_ferret_session **out);
+int cuot_ferret_session_info(const cuot_ferret_session *s, int64_t *M, int64_t *ot_limit, int64_t *round);
+int cuot_ferret_session_delta(const cuot_ferret_session *s, cuot_block *delta_d, void *stream);
+/* out_s_d, out_r_d: n blocks each (device); out_r = out_s ^ lsb(out_r) * Delta */
+int cuot_ferret_session_extend(cuot_ferret_session *s, cuot_block *out_s_d, cuot_block *out_r_d, void *stream);
+int cuot_ferret_session_free(cuot_ferret_session *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -778,3 +778,100 @@ void or_ferret_run(const or_ferret_cfg *cfg, int nthreads, orb *out_s, orb *out_
   free(Mb);
   free(cbits);
 }
+
+/* ---- iterating Ferret -------------------------------------------------------- */
+void or_ccrh(const orb *in, orb *out, size_t n) {
+  uint8_t rk[176];
+  orb zero = {0, 0};
+  or_aes_expand(&zero, rk);
+  orb *sg = (orb *)malloc((n ? n : 1) * sizeof(orb));
+  for (size_t i = 0; i < n; i++) {
+    sg[i].lo = in[i].hi;
+    sg[i].hi = in[i].hi ^ in[i].lo;
+  }
+  or_aes_ecb(rk, sg, out, n, 1);
+  for (size_t i = 0; i < n; i++) out[i] = bx(out[i], sg[i]);
+  free(sg);
+}
+
+int or_ferret_session_init(or_ferret_session *s, const or_ferret_cfg *cfg) {
+  memset(s, 0, sizeof(*s));
+  s->cfg = *cfg;
+  const int64_t npre = (int64_t)cfg->h * cfg->t;
+  s->M = cfg->k + npre + 128;
+  s->ot_limit = cfg->n - s->M;
+  if (s->ot_limit <= 0) return -1;
+  s->delta = derive_delta(&cfg->seed);
+  s->boot_s = (orb *)malloc((size_t)s->M * sizeof(orb));
+  s->boot_r = (orb *)malloc((size_t)s->M * sizeof(orb));
+  or_prg(&cfg->seed, 7, 0, (size_t)s->M, s->boot_s);
+  const int64_t cb = (s->M + 127) / 128;
+  orb *cbits = (orb *)malloc((size_t)cb * sizeof(orb));
+  or_prg(&cfg->seed, 8, 0, (size_t)cb, cbits);
+  const uint8_t *cbb = (const uint8_t *)cbits;
+  for (int64_t j = 0; j < s->M; j++) {
+    s->boot_s[j].lo &= ~1ull;
+    s->boot_r[j] = s->boot_s[j];
+    if ((cbb[j >> 3] >> (j & 7)) & 1) s->boot_r[j] = bx(s->boot_r[j], s->delta);
+  }
+  free(cbits);
+  return 0;
+}
+
+void or_ferret_session_extend(or_ferret_session *s, int nthreads, orb *out_s, orb *out_r) {
+  const or_ferret_cfg *cfg = &s->cfg;
+  const int64_t n = cfg->n, t = cfg->t, k = cfg->k;
+  const int h = cfg->h;
+  const int64_t npre = (int64_t)h * t;
+  orb D = s->delta;
+  /* pre-OT */
+  orb *P0 = (orb *)malloc((size_t)npre * sizeof(orb));
+  orb *P1 = (orb *)malloc((size_t)npre * sizeof(orb));
+  orb *Pr = (orb *)malloc((size_t)npre * sizeof(orb));
+  orb *kd = (orb *)malloc((size_t)npre * sizeof(orb));
+  uint8_t *bits = (uint8_t *)malloc((size_t)npre);
+  or_ccrh(s->boot_s, P0, (size_t)npre);
+  for (int64_t i = 0; i < npre; i++) kd[i] = bx(s->boot_s[i], D);
+  or_ccrh(kd, P1, (size_t)npre);
+  for (int64_t i = 0; i < npre; i++) bits[i] = (uint8_t)(s->boot_r[i].lo & 1);
+  or_ccrh(s->boot_r, Pr, (size_t)npre);
+  /* sender MPCOT */
+  orb *roots = (orb *)malloc((size_t)t * sizeof(orb));
+  or_prg(&cfg->seed, 1, (uint64_t)(s->round * t), (size_t)t, roots);
+  orb *ls = (orb *)malloc((size_t)npre * sizeof(orb));
+  orb *rs = (orb *)malloc((size_t)npre * sizeof(orb));
+  orb *ss = (orb *)malloc((size_t)t * sizeof(orb));
+  or_ferret_mpcot_sender(roots, t, h, D, n, out_s, ls, rs, ss, nthreads);
+  /* level-sum OTs through the pre-OT pads; the receiver unmasks its choice */
+  orb *csum = (orb *)malloc((size_t)npre * sizeof(orb));
+  for (int64_t i = 0; i < npre; i++) {
+    const orb pad0 = bx(ls[i], P0[i]), pad1 = bx(rs[i], P1[i]);
+    csum[i] = bx(bits[i] ? pad1 : pad0, Pr[i]);
+  }
+  or_ferret_mpcot_recver(t, h, bits, csum, ss, n, out_r, nthreads);
+  /* LPN with kk = bootstrap + 128 */
+  orb seed_lpn;
+  or_prg(&cfg->seed, 6, 0, 1, &seed_lpn);
+  or_ferret_lpn(out_s, n, s->boot_s + 128, k, seed_lpn, 10, cfg->ntasks, nthreads);
+  or_ferret_lpn(out_r, n, s->boot_r + 128, k, seed_lpn, 10, cfg->ntasks, nthreads);
+  /* refresh */
+  memcpy(s->boot_s, out_s + s->ot_limit, (size_t)s->M * sizeof(orb));
+  memcpy(s->boot_r, out_r + s->ot_limit, (size_t)s->M * sizeof(orb));
+  s->round++;
+  free(P0);
+  free(P1);
+  free(Pr);
+  free(kd);
+  free(bits);
+  free(roots);
+  free(ls);
+  free(rs);
+  free(ss);
+  free(csum);
+}
+
+void or_ferret_session_free(or_ferret_session *s) {
+  free(s->boot_s);
+  free(s->boot_r);
+  s->boot_s = s->boot_r = NULL;
+}

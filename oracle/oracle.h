@@ -145,6 +145,34 @@ typedef struct {
 /* out_s, out_r: n blocks; delta: 1 block */
 void or_ferret_run(const or_ferret_cfg *cfg, int nthreads, orb *out_s, orb *out_r, orb *delta);
 
+
+/* ---- iterating Ferret (FerretCOT::extend_f2k, ferret_cot.hpp:79-103) ------
+ * State: Delta and a bootstrap of M = k + h*t + 128 COTs per party (sender K
+ * with LSB 0, receiver Mr = K ^ c*Delta).  One extend:
+ *   pre-OT (preot.h:38-54): P0 = H(K[i]), P1 = H(K[i]^Delta) (sender), bits =
+ *     LSB(Mr[i]), Pr = H(Mr[i]) (receiver), i < h*t, H = the circular
+ *     correlation-robust hash H(x) = AES0(s(x)) ^ s(x), s(x) = (hi ^ lo, hi)
+ *     as (hi, lo) halves, AES0 = AES-128 under the all-zero key;
+ *   MPCOT: roots = PRG(seed, stream 1, start r*t); level sums sent as
+ *     m0 ^ P0, m1 ^ P1; the receiver's choice bits are bits[tree*h + level]
+ *     and its chosen sum pad_b ^ Pr; secret sum as in the single instance;
+ *   LPN with kk = bootstrap + 128 (ferret_cot.hpp:89), seed PRG(seed, 6, 0);
+ *   the last M outputs become the next bootstrap (ferret_cot.hpp:100).
+ * Initial bootstrap: K = PRG(seed, 7) & ~1, c = PRG(seed, 8) bits, M blocks. */
+typedef struct {
+  or_ferret_cfg cfg;
+  int64_t M, ot_limit, round;
+  orb delta;
+  orb *boot_s, *boot_r; /* M blocks each */
+} or_ferret_session;
+
+int or_ferret_session_init(or_ferret_session *s, const or_ferret_cfg *cfg);
+/* one extend: out_s, out_r receive all n outputs (the first ot_limit are the
+ * user's; the rest become the bootstrap) */
+void or_ferret_session_extend(or_ferret_session *s, int nthreads, orb *out_s, orb *out_r);
+void or_ferret_session_free(or_ferret_session *s);
+void or_ccrh(const orb *in, orb *out, size_t n);
+
 #ifdef __cplusplus
 }
 #endif

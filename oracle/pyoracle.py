@@ -36,6 +36,11 @@ class FERRETCFG(C.Structure):
     _fields_ = [("n", I64), ("k", I64), ("t", I64), ("h", INT), ("ntasks", INT), ("seed", ORB)]
 
 
+class FERRETSESSION(C.Structure):
+    _fields_ = [("cfg", FERRETCFG), ("M", I64), ("ot_limit", I64), ("round", I64), ("delta", ORB),
+                ("boot_s", P), ("boot_r", P)]
+
+
 def build(force: bool = False) -> str:
     if force or not os.path.exists(_LIB_PATH):
         subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
@@ -68,6 +73,11 @@ def lib():
         L.or_ferret_lpn_indices.argtypes = [I64, I64, ORB, INT, INT, P]
         L.or_sot_run.argtypes = [C.POINTER(SOTCFG), INT, P, P, P, P, P, P]
         L.or_ferret_run.argtypes = [C.POINTER(FERRETCFG), INT, P, P, P]
+        L.or_ferret_session_init.argtypes = [C.POINTER(FERRETSESSION), C.POINTER(FERRETCFG)]
+        L.or_ferret_session_init.restype = INT
+        L.or_ferret_session_extend.argtypes = [C.POINTER(FERRETSESSION), INT, P, P]
+        L.or_ferret_session_free.argtypes = [C.POINTER(FERRETSESSION)]
+        L.or_ccrh.argtypes = [P, P, C.c_size_t]
         _lib = L
     return _lib
 
@@ -259,3 +269,55 @@ def ferret_run(n, k, t, h, seed, ntasks=1, nthreads=1):
     s, r, d = blocks(n), blocks(n), blocks(1)
     lib().or_ferret_run(C.byref(cfg), nthreads, _p(s), _p(r), _p(d))
     return s, r, d[0]
+
+
+class FerretSession:
+    """The oracle's iterating Ferret (or_ferret_session_*): extend() returns the
+    n sender and receiver outputs of one round."""
+
+    def __init__(self, n, k, t, h, seed, ntasks=1, nthreads=1):
+        self.cfg = FERRETCFG(n, k, t, h, ntasks, as_orb(seed))
+        self.s = FERRETSESSION()
+        self.nthreads = nthreads
+        if lib().or_ferret_session_init(C.byref(self.s), C.byref(self.cfg)) != 0:
+            raise ValueError("n must exceed k + h*t + 128")
+        self.n = n
+
+    @property
+    def M(self):
+        return self.s.M
+
+    @property
+    def ot_limit(self):
+        return self.s.ot_limit
+
+    @property
+    def delta(self):
+        return np.array([self.s.delta.lo, self.s.delta.hi], dtype=np.uint64)
+
+    def bootstrap(self):
+        bs = np.ctypeslib.as_array(C.cast(self.s.boot_s, C.POINTER(C.c_uint64)), shape=(self.s.M, 2)).copy()
+        br = np.ctypeslib.as_array(C.cast(self.s.boot_r, C.POINTER(C.c_uint64)), shape=(self.s.M, 2)).copy()
+        return bs, br
+
+    def extend(self):
+        o_s, o_r = blocks(self.n), blocks(self.n)
+        lib().or_ferret_session_extend(C.byref(self.s), self.nthreads, _p(o_s), _p(o_r))
+        return o_s, o_r
+
+    def close(self):
+        if self.s.boot_s:
+            lib().or_ferret_session_free(C.byref(self.s))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ccrh(x):
+    x = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, 2)
+    out = blocks(x.shape[0])
+    lib().or_ccrh(_p(x), _p(out), x.shape[0])
+    return out

@@ -327,6 +327,28 @@ def main():
             "relation_ok": fok,
         }
 
+    # iterating Ferret session (FerretCOT extend loop) at the C2 size
+    session = None
+    if not args.no_ferret:
+        from cuot_amd.ferret import FerretCOT
+
+        fc = FerretCOT(ctx, n=10168320, k=452160, t=1270, h=13, seed=SEED)
+        fc.extend_f2k()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            fc.extend_f2k()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        sms = e0.elapsed_time(e1) / args.steps
+        lsb = (fc.out_r[:, 0] & 1).to(torch.bool)
+        sok = bool(torch.equal(torch.where(lsb.unsqueeze(1), fc.out_s ^ fc.Delta[0], fc.out_s), fc.out_r))
+        session = {"config": {"workload": "ferret session (extend loop)", "n": 10168320, "k": 452160, "t": 1270,
+                              "bootstrap_M": fc.M, "ot_limit": fc.ot_limit},
+                   "ms_per_extend": round(sms, 4), "value": round(fc.ot_limit / (sms / 1e3), 1),
+                   "unit": "usable COTs/s", "relation_ok": sok}
+        fc.close()
     # release the headline buffers before the other legs
     del ws, v, w, b
     torch.cuda.empty_cache()
@@ -378,6 +400,7 @@ def main():
         "ggm": {"aes_per_launch": aes_per_party, "ms_per_launch": round(ggm_ms, 4),
                 "aes_per_s": round(aes_per_party / (ggm_ms / 1e3), 1)},
         "ferret": ferret,
+        "ferret_session": session,
         "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu,

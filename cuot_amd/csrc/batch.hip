@@ -351,6 +351,10 @@ hipError_t launch_all(cuot_batch* b, uint32_t stages, hipStream_t st) {
       const SotPlan& p = b->sot[i];
       if ((e = cuot_launch_sparse(p.a_words, p.cfg.n, p.pos, (int)p.cfg.t, b->bits[i], st))) return e;
     }
+  if ((stages & CUOT_BATCH_STAGE_LPN) && !b->ferret_ids.empty()) {
+    for (int i : b->ferret_ids)
+      if ((e = b->fer[i].fill_kkp(st))) return e;
+  }
   if ((stages & CUOT_BATCH_STAGE_LPN) && !b->ferret_ids.empty())
     if ((e = cuot_launch_lpn_batch(ws + b->lpn_tab_off, (const int64_t*)(ws + b->lpn_start_off),
                                    (int)b->ferret_ids.size(), b->lpn_wgs, 2, st)))

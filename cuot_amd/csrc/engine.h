@@ -34,6 +34,7 @@ struct FerretPlan {
   blk *delta, *roots, *alpha;
   uint8_t* choice;
   blk *sums, *csum, *ssum, *lpn_seed, *K, *Mr, *cbits;
+  blk* kkp;  // K and Mr interleaved for the LPN gathers
 
   size_t carve(void* ws);
   int validate() const;
@@ -43,7 +44,8 @@ struct FerretPlan {
   hipError_t run_sender_sums(hipStream_t st);  // secret sum + receiver's chosen sums
   hipError_t run_mpcot_recver(blk* out_r, hipStream_t st);
   hipError_t run_mpcot(blk* out_s, blk* out_r, hipStream_t st);
-  void lpn_launch(blk* out_s, blk* out_r, LpnLaunch& l) const;
+  void lpn_launch(blk* out_s, blk* out_r, LpnLaunch& l) const;  // gathers from kkp (fill it first)
+  hipError_t fill_kkp(hipStream_t st);
   hipError_t run_lpn(blk* out_s, blk* out_r, hipStream_t st);
 };
 

@@ -152,6 +152,7 @@ size_t FerretPlan::carve(void* ws) {
   K = c.take<blk>(k);
   Mr = c.take<blk>(k);
   cbits = c.take<blk>((k + 127) / 128);
+  kkp = c.take<blk>(2 * k);
   return c.off + 256;
 }
 
@@ -243,9 +244,14 @@ void FerretPlan::lpn_launch(blk* out_s, blk* out_r, LpnLaunch& l) const {
   l.noise[1] = out_r;
   l.kk[1] = Mr;
   l.out[1] = out_r;
+  l.kkp = kkp;
 }
 
+hipError_t FerretPlan::fill_kkp(hipStream_t st) { return cuot_launch_interleave(K, Mr, cfg.k, kkp, st); }
+
 hipError_t FerretPlan::run_lpn(blk* out_s, blk* out_r, hipStream_t st) {
+  hipError_t e = fill_kkp(st);
+  if (e) return e;
   LpnLaunch l;
   lpn_launch(out_s, out_r, l);
   return cuot_launch_lpn(l, st);

@@ -47,9 +47,12 @@ struct LpnLaunch {
   int nroles;
   const blk* noise[2];
   const blk* kk[2];
+  const blk* kkp;  // optional: kk[0], kk[1] interleaved (2k blocks), one 32-byte gather for both roles
   blk* out[2];
 };
 hipError_t cuot_launch_lpn(const LpnLaunch& l, hipStream_t st);
+// out[2j] = a[j], out[2j+1] = b[j], j < k (proto.hip)
+hipError_t cuot_launch_interleave(const blk* a, const blk* b, int64_t k, blk* out, hipStream_t st);
 // batch form: per-instance argument records (cuot_lpn_args_bytes each, filled
 // by cuot_lpn_args_fill, which returns the instance's WG count or -1) and the
 // prefix of WG counts, both in device memory; all instances share nroles

@@ -33,9 +33,23 @@ struct LpnKArgs {
   int64_t units_total;
   const blk* noise[2];
   const blk* kk[2];
+  const blk* kkp;  // both roles' kk interleaved (kkp[2j], kkp[2j+1]) or NULL
   blk* out[2];
   int nroles;
 };
+
+// kk gathers: with the interleaved table one 32-byte read serves both roles
+template <int NR>
+CUOT_DEV void gather_kk(const LpnKArgs& a, uint32_t idx, blk* acc) {
+  if (NR == 2 && a.kkp) {
+    const blk* p = a.kkp + 2 * (int64_t)idx;
+    acc[0] = bxor(acc[0], p[0]);
+    acc[NR - 1] = bxor(acc[NR - 1], p[1]);
+  } else {
+#pragma unroll
+    for (int rr = 0; rr < NR; rr++) acc[rr] = bxor(acc[rr], a.kk[rr][idx]);
+  }
+}
 
 // number of __compute4 groups and __compute1 rows of a task of `rows` rows
 CUOT_DEV void task_shape(int64_t rows, int64_t& g4, int64_t& t1) {
@@ -74,8 +88,7 @@ CUOT_DEV void lpn_unit(const AesLds& aes, const LpnKArgs& a, int64_t u) {
       for (int j = 0; j < D; j++) {
         uint32_t idx = w[q * D + j] & a.mask;
         if (idx >= (uint32_t)a.k) idx -= (uint32_t)a.k;
-#pragma unroll
-        for (int rr = 0; rr < NR; rr++) acc[rr] = bxor(acc[rr], a.kk[rr][idx]);
+        gather_kk<NR>(a, idx, acc);
       }
 #pragma unroll
       for (int rr = 0; rr < NR; rr++) a.out[rr][i + q] = acc[rr];
@@ -98,8 +111,7 @@ CUOT_DEV void lpn_unit(const AesLds& aes, const LpnKArgs& a, int64_t u) {
 #pragma unroll
     for (int j = 0; j < D; j++) {
       const uint32_t idx = w[j] % (uint32_t)a.k;
-#pragma unroll
-      for (int rr = 0; rr < NR; rr++) acc[rr] = bxor(acc[rr], a.kk[rr][idx]);
+      gather_kk<NR>(a, idx, acc);
     }
 #pragma unroll
     for (int rr = 0; rr < NR; rr++) a.out[rr][i] = acc[rr];
@@ -170,6 +182,7 @@ bool fill_args(const LpnLaunch& l, LpnKArgs& a) {
   for (int r = 0; r < l.nroles; r++) {
     a.noise[r] = l.noise[r];
     a.kk[r] = l.kk[r];
+    a.kkp = l.nroles == 2 ? l.kkp : nullptr;
     a.out[r] = l.out[r];
   }
   return true;

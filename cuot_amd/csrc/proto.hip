@@ -99,7 +99,20 @@ __global__ void ferret_boot_kernel(blk* K, blk* Mr, const uint8_t* cbits, const 
   }
 }
 
+__global__ void interleave_kernel(const blk* __restrict__ a, const blk* __restrict__ b, int64_t k,
+                                  blk* __restrict__ out) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < k; j += (int64_t)gridDim.x * blockDim.x) {
+    out[2 * j] = a[j];
+    out[2 * j + 1] = b[j];
+  }
+}
+
 }  // namespace
+
+hipError_t cuot_launch_interleave(const blk* a, const blk* b, int64_t k, blk* out, hipStream_t st) {
+  hipLaunchKernelGGL(interleave_kernel, dim3(grid_for(k)), dim3(256), 0, st, a, b, k, out);
+  return hipGetLastError();
+}
 
 hipError_t cuot_launch_setup_fix(blk* delta, const blk* alpha_blocks, int64_t t, int h, uint8_t* choice,
                                  int64_t* pos, uint32_t* a_words, int64_t n, int64_t a_words_total,

@@ -88,7 +88,7 @@ struct cuot_ferret_session {
   FerretPlan plan;  // parameters, GGM/LPN argument builders
   int64_t npre = 0, M = 0, ot_limit = 0, round = 0;
   char* mem = nullptr;
-  blk *delta, *boot_s, *boot_r, *P0, *P1, *Pr, *roots, *sums, *csum, *ssum, *cbits;
+  blk *delta, *boot_s, *boot_r, *P0, *P1, *Pr, *roots, *sums, *csum, *ssum, *cbits, *kkp;
   uint8_t* bits;
 };
 
@@ -115,6 +115,7 @@ size_t carve_session(cuot_ferret_session* s, char* base) {
   s->csum = (blk*)take((size_t)t * h * 16);
   s->ssum = (blk*)take((size_t)t * 16);
   s->cbits = (blk*)take((size_t)((s->M + 127) / 128) * 16);
+  s->kkp = (blk*)take((size_t)2 * s->plan.cfg.k * 16);
   return off + 256;
 }
 
@@ -164,10 +165,12 @@ hipError_t extend(cuot_ferret_session* s, blk* out_s, blk* out_r, hipStream_t st
   r.fin = s->ssum;
   if ((e = cuot_launch_ggm(r, true, st))) return e;
   // 3. LPN with kk = bootstrap + 128
+  if ((e = cuot_launch_interleave(s->boot_s + 128, s->boot_r + 128, cfg.k, s->kkp, st))) return e;
   LpnLaunch l;
   s->plan.lpn_launch(out_s, out_r, l);
   l.kk[0] = s->boot_s + 128;
   l.kk[1] = s->boot_r + 128;
+  l.kkp = s->kkp;
   if ((e = cuot_launch_lpn(l, st))) return e;
   // 4. refresh the bootstrap from the last M outputs
   if ((e = hipMemcpyAsync(s->boot_s, out_s + s->ot_limit, (size_t)s->M * 16, hipMemcpyDeviceToDevice, st))) return e;

@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--no-ferret", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--sot-n-log", type=int, default=24)
+    ap.add_argument("--sequential", action="store_true",
+                    help="time staged (non-overlapped) instances for the headline value")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (64-instance) leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-requests", type=int, default=2000)
@@ -242,32 +244,46 @@ def main():
     d = torch.empty((1, 2), dtype=torch.int64, device="cuda")
     stages = cuot_amd.SOT_STAGES
 
-    def sot_step(events=None):
+    def sot_staged(events):
         for i, (_, bit) in enumerate(stages):
-            if events is not None:
-                events[i].record(stream)
+            events[i].record(stream)
             ctx.sot_run_stages(cfg, ws, v, w, b, d, bit, stream)
-        if events is not None:
-            events[-1].record(stream)
+        events[-1].record(stream)
+
+    # timed region 1 (value): whole instances; independent stages of the two
+    # roles overlap on two streams (staged and sequential with --sequential)
+    def sot_whole():
+        if args.sequential:
+            ctx.sot_run_stages(cfg, ws, v, w, b, d, cuot_amd.SOT_ALL, stream)
+        else:
+            ctx.sot_run(cfg, ws, v, w, b, d, stream)
 
     for _ in range(args.warmup):
-        sot_step()
+        sot_whole()
     torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sot_whole()
     torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+
+    # timed region 2 (kernels): K staged, sequential instances with HIP events
+    # between stages and around every kernel launch on its stream, so that each
+    # kernel's duration is its own (no other work shares the GPU)
     lib = ctx.lib
     import ctypes as C
 
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)] for _ in range(args.steps)]
     cuot_amd.check(lib.cuot_timing_begin(ctx.handle), "cuot_timing_begin")
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        sot_step(ev[s])
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    recs = (cuot_amd._lib.LaunchTime * 4096)()
+    for s_ in range(args.steps):
+        sot_staged(evs[s_])
+    recs = (cuot_amd._lib.LaunchTime * 8192)()
     nrec = C.c_int()
-    cuot_amd.check(lib.cuot_timing_end(ctx.handle, recs, 4096, C.byref(nrec)), "cuot_timing_end")
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    cuot_amd.check(lib.cuot_timing_end(ctx.handle, recs, 8192, C.byref(nrec)), "cuot_timing_end")
+    torch.cuda.synchronize()
+    stage_ms = {name: round(sum(evs[s_][i].elapsed_time(evs[s_][i + 1]) for s_ in range(args.steps)) / args.steps, 4)
+                for i, (name, _) in enumerate(stages)}
     kernels = {}
     for r in recs[: nrec.value]:
         k = kernels.setdefault(r.kernel.decode(), {"launches": 0, "ms": 0.0, "bytes": 0.0})
@@ -280,9 +296,8 @@ def main():
                          "algorithmic_GBps": round(k["bytes"] / (k["ms"] / 1e3) / 1e9, 1) if k["ms"] > 0 else None}
                   for name, k in kernels.items()}
     dom_name, dom = max(kernels.items(), key=lambda kv: kv[1]["ms"])
-    stage_ms = {}
-    for i, (name, _) in enumerate(stages):
-        stage_ms[name] = round(sum(ev[s][i].elapsed_time(ev[s][i + 1]) for s in range(args.steps)) / args.steps, 4)
+    sot_whole()
+    torch.cuda.synchronize()
     ok = relation_ok(torch, v, w, b, d, n)
 
     qc_ms = (stage_ms["qc_sender"] + stage_ms["qc_recver"]) / 2
@@ -380,7 +395,8 @@ def main():
         "dtype": "u32",
         "data": "synthetic (all inputs derived from one 128-bit seed by AES-CTR)",
         "config": {"workload": "plain SOT", "n": n, "t": t, "h": h, "leaves": 2 * n, "qc_scaler": 2,
-                   "form": "plain, one launch sequence per stage, both roles"},
+                   "form": ("plain single instance, both roles, stages in sequence" if args.sequential else
+                         "plain single instance, both roles; independent stages overlap on two streams")},
         "relation_ok": ok,
         "stage_ms": stage_ms,
         "roofline": {

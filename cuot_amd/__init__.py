@@ -278,6 +278,8 @@ def make_instance_struct(kind: int, n: int, t: int, h: int, k: int, seed, ntasks
                     seed=to_block(seed))
 
 
+SOT_ALL = 127
+FERRET_ALL = 15
 SOT_STAGES = [("setup", 1), ("qc_prepare", 2), ("ggm_sender", 4), ("ggm_recver", 8), ("qc_sender", 16),
               ("qc_recver", 32), ("sparse", 64)]
 FERRET_STAGES = [("setup", 1), ("ggm_sender", 2), ("ggm_recver", 4), ("lpn", 8)]

@@ -15,6 +15,7 @@ struct SotPlan {
   uint32_t* ahat;
   blk *leaves_s, *leaves_r;
   char* qc_scratch;
+  char* qc_scratch2;       // second encode's scratch (both parties' encodes overlap)
   bool shared_qc = false;  // batch: the QC scratch lives outside the instance
 
   size_t carve(void* ws);  // returns bytes needed; fills pointers if ws != NULL

@@ -55,6 +55,7 @@ size_t SotPlan::carve(void* ws) {
   leaves_s = c.take<blk>(2 * n);
   leaves_r = c.take<blk>(2 * n);
   qc_scratch = shared_qc ? nullptr : c.take<char>(cuot_qc_scratch_bytes_for(n));
+  qc_scratch2 = shared_qc ? nullptr : c.take<char>(cuot_qc_scratch_bytes_for(n));
   return c.off + 256;
 }
 
@@ -296,9 +297,43 @@ extern "C" int cuot_sot_run_stages(cuot_ctx* ctx, const cuot_sot_config* cfg, vo
   return CUOT_OK;
 }
 
+// Whole instance with the independent stages overlapped on two streams (the
+// caller's and the context's second stream, joined before returning):
+//   caller: setup -> sender GGM + masked rows -> [A^] -> sender QC -> [join]
+//   second: [setup] -> A^ prepare -> sparse -> [rows] -> receiver GGM -> receiver QC
+// The two encodes use separate scratch.  Same bytes as the staged run.
 extern "C" int cuot_sot_run(cuot_ctx* ctx, const cuot_sot_config* cfg, void* ws_d, size_t ws_bytes, cuot_block* v_d,
                             cuot_block* w_d, uint8_t* b_d, cuot_block* delta_d, void* stream) {
-  return cuot_sot_run_stages(ctx, cfg, ws_d, ws_bytes, v_d, w_d, b_d, delta_d, CUOT_SOT_STAGE_ALL, stream);
+  CUOT_CHECK_ARG(ctx && cfg, "ctx/cfg NULL");
+  SotPlan p;
+  p.cfg = *cfg;
+  int rc = p.validate();
+  if (rc) return rc;
+  CUOT_CHECK_ARG(ws_d && v_d && w_d && b_d, "buffer is NULL");
+  CUOT_CHECK_ARG(ws_bytes >= p.carve(nullptr), "workspace too small (cuot_sot_workspace_bytes)");
+  p.carve(ws_d);
+  CUOT_HIP_TRY(hipSetDevice(ctx->device));
+  CUOT_HIP_TRY(ctx->ensure_aux());
+  CUOT_HIP_TRY(cuot_qc_configure());
+  hipStream_t st = ctx->st(stream), ax = ctx->aux;
+  hipEvent_t e_setup = ctx->ev[0], e_prep = ctx->ev[1], e_rows = ctx->ev[2], e_done = ctx->ev[3];
+  CUOT_HIP_TRY(p.run_setup(st));
+  if (delta_d) CUOT_HIP_TRY(hipMemcpyAsync(delta_d, p.delta, 16, hipMemcpyDeviceToDevice, st));
+  CUOT_HIP_TRY(hipEventRecord(e_setup, st));
+  CUOT_HIP_TRY(hipStreamWaitEvent(ax, e_setup, 0));
+  CUOT_HIP_TRY(p.run_prepare(ax));
+  CUOT_HIP_TRY(hipEventRecord(e_prep, ax));
+  CUOT_HIP_TRY(cuot_launch_sparse(p.a_words, p.cfg.n, p.pos, (int)p.cfg.t, (uint32_t*)b_d, ax));
+  CUOT_HIP_TRY(p.run_expand_sender(st));
+  CUOT_HIP_TRY(hipEventRecord(e_rows, st));
+  CUOT_HIP_TRY(hipStreamWaitEvent(ax, e_rows, 0));
+  CUOT_HIP_TRY(p.run_expand_recver(ax));
+  CUOT_HIP_TRY(cuot_launch_qc_encode(p.ahat, p.cfg.n, p.leaves_r, (blk*)w_d, p.qc_scratch2, ax));
+  CUOT_HIP_TRY(hipEventRecord(e_done, ax));
+  CUOT_HIP_TRY(hipStreamWaitEvent(st, e_prep, 0));
+  CUOT_HIP_TRY(cuot_launch_qc_encode(p.ahat, p.cfg.n, p.leaves_s, (blk*)v_d, p.qc_scratch, st));
+  CUOT_HIP_TRY(hipStreamWaitEvent(st, e_done, 0));
+  return CUOT_OK;
 }
 
 extern "C" size_t cuot_ferret_workspace_bytes(const cuot_ferret_config* cfg) {

@@ -292,6 +292,24 @@ struct FftPassArgs {
   int64_t XS_words;  // instance stride of X in 32-bit words (blockIdx.y)
 };
 
+// The top forward layer (k = m - 1, tile-relative s = R - 1) of the first
+// pass: the upper half of the padded input is zero, so u += c*0 leaves u and
+// v += u makes v a copy of u.
+__device__ __forceinline__ void fft_copy_top(int R, int C, int lgC, uint32_t* lds) {
+  const int s = R - 1;
+  const int ntask = ((1 << R) >> 1) * C * 4;
+  for (int tau = threadIdx.x; tau < ntask; tau += blockDim.x) {
+    const int w = tau & 3;
+    const int c = (tau >> 2) & (C - 1);
+    const int bp = (tau >> 2) >> lgC;  // < 2^s: tu = bp, tv = bp + 2^s
+    const uint32_t* pu = lds + (bp * C + c) * FFT_STRIDE + w;
+    uint32_t* pv = lds + ((bp | (1 << s)) * C + c) * FFT_STRIDE + w;
+#pragma unroll
+    for (int p = 0; p < 32; p++) pv[4 * p] = pu[4 * p];
+  }
+  __syncthreads();
+}
+
 __device__ __forceinline__ void fft_layer(int lo, int R, int C, int lgC, uint32_t* lds, int64_t hib,
                                                   int64_t cb, int k, bool inv) {
   const uint32_t* ctab = lds + (C << R) * FFT_STRIDE;
@@ -421,7 +439,10 @@ __global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(2))
       } else {
         const bool inv = step >= s_fwd;
         const int k = inv ? a.lo + (step - s_fwd - s_pw) : a.khi - step;
-        fft_layer(a.lo, a.R, a.C, a.lgC, lds, hib, cb, k, inv);
+        if (a.first && step == 0)
+          fft_copy_top(a.R, a.C, a.lgC, lds);  // top layer on a zero upper half: v = u
+        else
+          fft_layer(a.lo, a.R, a.C, a.lgC, lds, hib, cb, k, inv);
       }
     }
     uint4* Y4 = (uint4*)a.X;

@@ -9,6 +9,7 @@
 //     bit k%8, LSB-first within bytes -- done as 32x32 in-register transposes
 //     across 32 lanes, staged through LDS so both sides are coalesced.
 #include "kernels.h"
+#include "transpose32.h"
 
 namespace {
 
@@ -57,20 +58,7 @@ __global__ void sparse_encode_kernel(const uint32_t* __restrict__ a, int64_t n, 
 }
 
 // ---- transpose ---------------------------------------------------------------
-CUOT_DEV uint32_t transpose32(uint32_t x, int lane) {
-  const uint32_t masks[5] = {0x0000FFFFu, 0x00FF00FFu, 0x0F0F0F0Fu, 0x33333333u, 0x55555555u};
-#pragma unroll
-  for (int s = 0; s < 5; s++) {
-    const int j = 16 >> s;
-    const uint32_t m = masks[s];
-    const uint32_t xp = (uint32_t)__shfl_xor((int)x, j, 64);
-    if ((lane & j) == 0)
-      x = (x & m) | ((xp & m) << j);
-    else
-      x = ((xp >> j) & m) | (x & ~m);
-  }
-  return x;
-}
+CUOT_DEV uint32_t transpose32(uint32_t x, int lane) { return cuot_transpose32(x, lane); }
 
 // blocks -> rows.  WG: 256 lanes, 1024 blocks (32 tiles of 32 blocks).
 // rows: 128 rows of N bits, row stride N/32 words.  blockIdx.y selects a plane

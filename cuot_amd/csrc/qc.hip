@@ -31,6 +31,7 @@
 #include "conv_fast.h"
 #include "gf32.h"
 #include "kernels.h"
+#include "transpose32.h"
 
 namespace {
 
@@ -41,6 +42,7 @@ constexpr int FFT_LOGTC = 7;
 constexpr int FFT_STRIDE = 132;  // LDS words per element (32 planes x 4 words + pad)
 constexpr int FFT_THREADS = 256;  // one (butterfly, word) task per lane per layer
 constexpr int FFT_LD = 16;  // 16-byte loads in flight per lane (tile staging)
+
 constexpr int FFT_WGS = 512;  // FFT pass grid: two resident WGs on each of the 256 CUs
 constexpr int CR_LD = 8;    // same for the bit-row conversion tiles
 // the row-layout conversion handles strings of 2^5 .. 2^20 coefficients (128 KB LDS)
@@ -234,9 +236,14 @@ struct BarrierSync {
   __device__ void operator()() const { __syncthreads(); }
 };
 
+// fold (inverse mode, odd tiles = the upper half q of a row): after the
+// conversion, q is replaced by q ^ XOR_{e in E} rot(q, e), E = the lower terms
+// of s_mp, rotations cyclic over the 2^mp-bit string.  The lower half r then
+// satisfies r ^ T(q) = (F mod X^(2^mp) + 1) for F = r + s_mp q, the product
+// folded at half its length (s_mp = X^(2^mp) + sum_E X^e).
 __global__ __launch_bounds__(1024) void qc_conv_rows(uint32_t* __restrict__ Bw, int64_t tile_words, int64_t ntiles,
                                                      int64_t tiles_per_inst, int64_t inst_words, int mp,
-                                                     int inverse) {
+                                                     int inverse, int fold) {
   extern __shared__ uint32_t sw[];
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t* g = Bw + (tile / tiles_per_inst) * inst_words + (tile % tiles_per_inst) * tile_words;
@@ -263,7 +270,38 @@ __global__ __launch_bounds__(1024) void qc_conv_rows(uint32_t* __restrict__ Bw, 
     }
     __syncthreads();
     cbt_convert(sw, mp, inverse != 0, threadIdx.x, blockDim.x, BarrierSync());
-    if ((tile_words & 3) == 0) {
+    if (fold && ((tile % tiles_per_inst) & 1)) {
+      // tile_words is a power of two and a multiple of 4 on this path
+      uint32_t e[16];
+      const int ne = cb_lower_terms(mp, e);
+      const int64_t wm = tile_words - 1;
+      uint4* g4 = (uint4*)g;
+      const uint4* s4 = (const uint4*)sw;
+      for (int f = threadIdx.x; f < (int)(tile_words >> 2); f += blockDim.x) {
+        const int64_t w = 4 * (int64_t)f;
+        uint4 acc = s4[f];
+        for (int q = 0; q < ne; q++) {
+          const int64_t eo = e[q] >> 5;
+          const int eb = (int)(e[q] & 31);
+          uint4 c4;
+          if ((eo & 3) == 0) {
+            c4 = s4[((w - eo) & wm) >> 2];
+          } else {
+            c4 = make_uint4(sw[(w - eo) & wm], sw[(w - eo + 1) & wm], sw[(w - eo + 2) & wm], sw[(w - eo + 3) & wm]);
+          }
+          if (eb) {
+            const uint32_t pv = sw[(w - eo - 1) & wm];
+            acc.x ^= (c4.x << eb) | (pv >> (32 - eb));
+            acc.y ^= (c4.y << eb) | (c4.x >> (32 - eb));
+            acc.z ^= (c4.z << eb) | (c4.y >> (32 - eb));
+            acc.w ^= (c4.w << eb) | (c4.z >> (32 - eb));
+          } else {
+            acc = x4(acc, c4);
+          }
+        }
+        g4[f] = acc;
+      }
+    } else if ((tile_words & 3) == 0) {
       const int n4 = (int)(tile_words >> 2);
       uint4* g4 = (uint4*)g;
       for (int f = threadIdx.x; f < n4; f += blockDim.x) g4[f] = ((const uint4*)sw)[f];
@@ -601,6 +639,86 @@ __global__ void qc_output(const uint4* __restrict__ X, int64_t M, int m, int64_t
   }
 }
 
+// Output from the row layout after the folded inverse conversion (n = 16 h
+// chunks, h = M/2): G_p = r_p ^ T(q_p) is the product folded mod x^n - 1 for
+// coefficient plane p, and
+//   y[16c + b] = x0[16c + b] ^ G_b[c] ^ G_{b+16}[(c - 1) mod h]   (G_31 = 0),
+// the second term being the overlap of chunk c-1's upper 15 planes.
+// Step 1 (qc_rows_fold): per plane pair b and a tile of 32 words (1024
+// chunks) of every row, combine the four row strings in LDS and bit-transpose
+// them into the element layout D[b][c] (16 planes of h elements).
+// Step 2 (qc_output_planes): y[16c + b] = x0[16c + b] ^ D[b][c], interleaving
+// the 16 planes through LDS so that both sides are contiguous.
+__global__ __launch_bounds__(256) void qc_rows_fold(const uint32_t* __restrict__ Bw, int64_t hw, int64_t inst_words,
+                                                    uint4* __restrict__ D, int64_t dps) {
+  __shared__ uint32_t sm[128][33];
+  const int b = (int)blockIdx.y;
+  const uint32_t* rb = Bw + (int64_t)b * 128 * (2 * hw);
+  const uint32_t* rh = Bw + (int64_t)(b + 16) * 128 * (2 * hw);
+  D += (int64_t)b * dps;
+  const int lane = threadIdx.x & 31;
+  const int grp = threadIdx.x >> 5;
+  const int64_t ntile = hw / 32;
+  for (int64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
+    const int64_t w0 = t * 32;
+    // 128 rows x 32 words: thread f -> row f >> 3, words 4 (f & 7) .. + 3
+    for (int f = threadIdx.x; f < 128 * 8; f += 256) {
+      const int row = f >> 3, wq = (f & 7) * 4;
+      const int64_t ro = (int64_t)row * (2 * hw) + w0 + wq;
+      const uint4 a = x4(*(const uint4*)(rb + ro), *(const uint4*)(rb + ro + hw));
+      uint4 h = z4();
+      uint32_t hp = 0;
+      if (b < 15) {
+        h = x4(*(const uint4*)(rh + ro), *(const uint4*)(rh + ro + hw));
+        const int64_t rp = (int64_t)row * (2 * hw) + ((w0 + wq - 1) & (hw - 1));
+        hp = rh[rp] ^ rh[rp + hw];
+      }
+      uint32_t* S = &sm[row][wq];
+      S[0] = a.x ^ (h.x << 1) ^ (hp >> 31);
+      S[1] = a.y ^ (h.y << 1) ^ (h.x >> 31);
+      S[2] = a.z ^ (h.z << 1) ^ (h.y >> 31);
+      S[3] = a.w ^ (h.w << 1) ^ (h.z >> 31);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int tl = grp * 4 + r;
+      uint4 v;
+      v.x = cuot_transpose32(sm[0 * 32 + lane][tl], lane);
+      v.y = cuot_transpose32(sm[1 * 32 + lane][tl], lane);
+      v.z = cuot_transpose32(sm[2 * 32 + lane][tl], lane);
+      v.w = cuot_transpose32(sm[3 * 32 + lane][tl], lane);
+      D[(w0 + tl) * 32 + lane] = v;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void qc_output_planes(const uint4* __restrict__ D, int64_t dps, int64_t h,
+                                                        const QcIo* __restrict__ io, QcIo io1) {
+  __shared__ uint4 sm[16][65];
+  const QcIo q = io ? io[blockIdx.y] : io1;
+  const int64_t ntile = h / 64;
+  for (int64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
+    const int64_t c0 = t * 64;
+    // D[b][c0 .. c0 + 64) for 16 planes: 1024 elements, 4 per thread
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int f = (int)threadIdx.x + u * 256;
+      const int bb = f >> 6, cc = f & 63;
+      sm[bb][cc] = D[(int64_t)bb * dps + c0 + cc];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int f = (int)threadIdx.x + u * 256;  // block 16 c0 + f: chunk f >> 4, plane f & 15
+      const int64_t i = 16 * c0 + f;
+      q.y[i] = x4(q.x[i], sm[f & 15][f >> 4]);
+    }
+    __syncthreads();
+  }
+}
+
 unsigned grid_for(int64_t total, int64_t cap = 1 << 16) {
   int64_t g = (total + 255) / 256;
   if (g > cap) g = cap;
@@ -723,14 +841,14 @@ hipError_t conv_forward(Bufs b, int64_t M, int P, int mp, hipStream_t st) {
   const int64_t tpi = (int64_t)P * 128;
   cuot_rec_begin(st, "qc_conv_rows", 2.0 * (double)tpi * b.B * row_words * 4);
     hipLaunchKernelGGL(qc_conv_rows, dim3(capg(tpi * b.B, 4096)), dim3(1024), (size_t)row_words * 4, st, b.Bw,
-                     row_words, tpi * b.B, tpi, inst_words, mp, 0);
+                     row_words, tpi * b.B, tpi, inst_words, mp, 0, 0);
   cuot_rec_end(st);
   return cuot_launch_transpose_planes2(b.Bw, Mp, b.X, true, b.B * P, P, inst_words, 128 * row_words, b.XS, M, st);
 }
 
 // inverse conversion levels 0..mtop-1 of planes [0, P) of the full polynomial
 // (2^(mtop+1) elements): each half string is converted on its own
-hipError_t conv_inverse(Bufs b, int64_t M, int P, int mtop, hipStream_t st) {
+hipError_t conv_inverse(Bufs b, int64_t M, int P, int mtop, bool fold, hipStream_t st) {
   if (!row_path(mtop)) return conv_inverse_elem(b, M, P, mtop, st);
   const int64_t row_words = M / 32;
   const int64_t inst_words = b.XS * 4;
@@ -741,8 +859,9 @@ hipError_t conv_inverse(Bufs b, int64_t M, int P, int mtop, hipStream_t st) {
   const int64_t tpi = (int64_t)P * 128 * 2;
   cuot_rec_begin(st, "qc_conv_rows", 2.0 * (double)tpi * b.B * tile_words * 4);
     hipLaunchKernelGGL(qc_conv_rows, dim3(capg(tpi * b.B, 8192)), dim3(1024), (size_t)tile_words * 4, st, b.Bw,
-                     tile_words, tpi * b.B, tpi, inst_words, mtop, 1);
+                     tile_words, tpi * b.B, tpi, inst_words, mtop, 1, fold ? 1 : 0);
   cuot_rec_end(st);
+  if (fold) return hipGetLastError();  // the output is read from the row layout
   return cuot_launch_transpose_planes2(b.Bw, M, b.X, true, b.B * P, P, inst_words, 128 * row_words, b.XS, M, st);
 }
 
@@ -841,7 +960,21 @@ hipError_t encode_batch(const QcIo* io, QcIo io1, const uint32_t* ahat1, const u
   int khi;
   if ((e = fft_forward(b, m, ahat1, ahats, true, &khi, st))) return e;
   if ((e = fft_inverse_upper(b, m, khi + 1, st))) return e;
-  if ((e = conv_inverse(b, M, 31, m - 1, st))) return e;
+  // n = 16 * M/2 (n a power of two): fold in the row layout and write y from it
+  const bool fold = !io && io1.n == 8 * M && m >= 11 && row_path(m - 1) && B == 1;
+  if ((e = conv_inverse(b, M, 31, m - 1, fold, st))) return e;
+  if (fold) {
+    const int64_t h = M >> 1, hw = h / 32;
+    cuot_rec_begin(st, "qc_rows_fold", 31.0 * 16.0 * (double)M + 16.0 * 16.0 * (double)h);
+    hipLaunchKernelGGL(qc_rows_fold, dim3(capg(hw / 32, 1024), 16), dim3(256), 0, st, (const uint32_t*)b.Bw, hw,
+                       b.XS * 4, (uint4*)b.X, M);
+    cuot_rec_end(st);
+    cuot_rec_begin(st, "qc_output_planes", 48.0 * (double)io1.n);
+    hipLaunchKernelGGL(qc_output_planes, dim3(capg(h / 64, 8192), 1), dim3(256), 0, st, (const uint4*)b.X, M, h, io,
+                       io1);
+    cuot_rec_end(st);
+    return hipGetLastError();
+  }
   const int64_t nmax = io ? (M / 2) * 16 : io1.n;  // grid covers the largest n of the class
   cuot_rec_begin(st, "qc_output", 32.0 * (double)nmax * B);
     hipLaunchKernelGGL(qc_output, dim3(grid_for(nmax), (unsigned)B), dim3(256), 0, st, (const uint4*)b.X, M, m, b.XS,

@@ -130,6 +130,13 @@ SIGNATURES = {
     "cuot_ferret_session_delta": (INT, [P, P, P]),
     "cuot_ferret_session_extend": (INT, [P, P, P, P]),
     "cuot_ferret_session_free": (INT, [P]),
+    "cuot_ferret_session_extend_stages": (INT, [P, P, P, INT, P]),
+    "cuot_ferret_state_size": (I64, [C.POINTER(FerretConfig)]),
+    "cuot_ferret_session_assemble_state": (INT, [P, INT, P, I64]),
+    "cuot_ferret_session_disassemble_state": (INT, [P, C.POINTER(FerretConfig), I64, P, P, I64, C.POINTER(P)]),
+    "cuot_ferret_session_write_pre_file": (INT, [P, INT, C.c_char_p]),
+    "cuot_ferret_session_read_pre_files": (INT, [P, C.POINTER(FerretConfig), I64, C.c_char_p, C.c_char_p,
+                                                 C.POINTER(P)]),
     "cuot_serve_replay": (INT, [P, C.POINTER(ServerConfig), C.POINTER(Instance), C.POINTER(C.c_double), INT,
                                 C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(I64)]),
 }

@@ -14,7 +14,10 @@
 //   4. the last M outputs become the next bootstrap (ferret_cot.hpp:100).
 // H(x) = AES0(s(x)) ^ s(x) with s(x) = (hi ^ lo, hi) as (hi, lo) halves and
 // AES0 = AES-128 under the all-zero key.
+#include <stdio.h>
 #include <string.h>
+
+#include <vector>
 
 #include "aes_dev.h"
 #include "ctx.h"
@@ -87,6 +90,7 @@ struct cuot_ferret_session {
   cuot_ctx* ctx = nullptr;
   FerretPlan plan;  // parameters, GGM/LPN argument builders
   int64_t npre = 0, M = 0, ot_limit = 0, round = 0;
+  bool seeded = false;  // the MPCOT half of the current extend has run
   char* mem = nullptr;
   blk *delta, *boot_s, *boot_r, *P0, *P1, *Pr, *roots, *sums, *csum, *ssum, *cbits, *kkp;
   uint8_t* bits;
@@ -119,7 +123,8 @@ size_t carve_session(cuot_ferret_session* s, char* base) {
   return off + 256;
 }
 
-hipError_t extend(cuot_ferret_session* s, blk* out_s, blk* out_r, hipStream_t st) {
+// the first half of an extend (emp-ot's SeedExp event): pre-OT and MPCOT
+hipError_t extend_seed(cuot_ferret_session* s, blk* out_s, blk* out_r, hipStream_t st) {
   const cuot_ferret_config& cfg = s->plan.cfg;
   hipError_t e;
   // 1. pre-OT through the hash
@@ -163,7 +168,13 @@ hipError_t extend(cuot_ferret_session* s, blk* out_s, blk* out_r, hipStream_t st
   r.choice = s->bits;
   r.csum = s->csum;
   r.fin = s->ssum;
-  if ((e = cuot_launch_ggm(r, true, st))) return e;
+  return cuot_launch_ggm(r, true, st);
+}
+
+// the second half (emp-ot's LPN event): LPN and the bootstrap refresh
+hipError_t extend_lpn(cuot_ferret_session* s, blk* out_s, blk* out_r, hipStream_t st) {
+  const cuot_ferret_config& cfg = s->plan.cfg;
+  hipError_t e;
   // 3. LPN with kk = bootstrap + 128
   if ((e = cuot_launch_interleave(s->boot_s + 128, s->boot_r + 128, cfg.k, s->kkp, st))) return e;
   LpnLaunch l;
@@ -181,8 +192,10 @@ hipError_t extend(cuot_ferret_session* s, blk* out_s, blk* out_r, hipStream_t st
 
 }  // namespace
 
-extern "C" int cuot_ferret_session_new(cuot_ctx* ctx, const cuot_ferret_config* cfg, cuot_ferret_session** out) {
-  CUOT_CHECK_ARG(ctx && cfg && out, "NULL argument");
+namespace {
+
+// validated parameters and one device allocation carved into the state buffers
+int session_alloc(cuot_ctx* ctx, const cuot_ferret_config* cfg, cuot_ferret_session** out) {
   *out = nullptr;
   cuot_ferret_session* s = new cuot_ferret_session();
   s->ctx = ctx;
@@ -209,6 +222,63 @@ extern "C" int cuot_ferret_session_new(cuot_ctx* ctx, const cuot_ferret_config* 
     return CUOT_ERR_NOMEM;
   }
   carve_session(s, s->mem);
+  *out = s;
+  return CUOT_OK;
+}
+
+void session_destroy(cuot_ferret_session* s) {
+  if (s->mem) hipFree(s->mem);
+  delete s;
+}
+
+// ---- state blob and pre-OT file (emp-ot layouts) ------------------------------
+// party ids as in emp-ot: ALICE = 1 (sender, holds Delta), BOB = 2 (receiver)
+constexpr int64_t kAlice = 1, kBob = 2;
+
+int64_t state_bytes(int64_t M) { return 4 * (int64_t)sizeof(int64_t) + 16 + 16 * M; }
+
+// copies one party's Delta (zero for the receiver) and bootstrap to the host
+// after all work on the device has finished
+int export_party(const cuot_ferret_session* s, int64_t party, blk* delta_h, void* boot_h) {
+  CUOT_HIP_TRY(hipSetDevice(s->ctx->device));
+  CUOT_HIP_TRY(hipDeviceSynchronize());
+  if (party == kAlice) {
+    CUOT_HIP_TRY(hipMemcpy(delta_h, s->delta, 16, hipMemcpyDeviceToHost));
+  } else {
+    memset(delta_h, 0, 16);
+  }
+  CUOT_HIP_TRY(hipMemcpy(boot_h, party == kAlice ? s->boot_s : s->boot_r, (size_t)s->M * 16, hipMemcpyDeviceToHost));
+  return CUOT_OK;
+}
+
+// a session at `round` from both parties' Delta and bootstraps (host)
+int import_session(cuot_ctx* ctx, const cuot_ferret_config* cfg, int64_t round, const void* delta_h,
+                   const void* boot_s_h, const void* boot_r_h, cuot_ferret_session** out) {
+  CUOT_CHECK_ARG(round >= 0, "round must be >= 0");
+  cuot_ferret_session* s = nullptr;
+  int rc = session_alloc(ctx, cfg, &s);
+  if (rc) return rc;
+  s->round = round;
+  hipError_t e = hipMemcpy(s->delta, delta_h, 16, hipMemcpyHostToDevice);
+  if (!e) e = hipMemcpy(s->boot_s, boot_s_h, (size_t)s->M * 16, hipMemcpyHostToDevice);
+  if (!e) e = hipMemcpy(s->boot_r, boot_r_h, (size_t)s->M * 16, hipMemcpyHostToDevice);
+  if (e) {
+    session_destroy(s);
+    CUOT_HIP_TRY(e);
+  }
+  *out = s;
+  return CUOT_OK;
+}
+
+bool read_exact(FILE* f, void* dst, size_t bytes) { return fread(dst, 1, bytes, f) == bytes; }
+
+}  // namespace
+
+extern "C" int cuot_ferret_session_new(cuot_ctx* ctx, const cuot_ferret_config* cfg, cuot_ferret_session** out) {
+  CUOT_CHECK_ARG(ctx && cfg && out, "NULL argument");
+  cuot_ferret_session* s = nullptr;
+  int rc = session_alloc(ctx, cfg, &s);
+  if (rc) return rc;
   // Delta and the initial bootstrap from the seed (streams 0, 7, 8)
   uint32_t rk[44];
   cuot_aes_expand_host((const uint8_t*)&cfg->seed, rk);
@@ -220,11 +290,99 @@ extern "C" int cuot_ferret_session_new(cuot_ctx* ctx, const cuot_ferret_config* 
   if (!e) e = cuot_launch_ferret_boot(s->boot_s, s->boot_r, (const uint8_t*)s->cbits, s->delta, s->M, st);
   if (!e) e = hipStreamSynchronize(st);
   if (e) {
-    hipFree(s->mem);
-    delete s;
+    session_destroy(s);
     CUOT_HIP_TRY(e);
   }
   *out = s;
+  return CUOT_OK;
+}
+
+extern "C" int64_t cuot_ferret_state_size(const cuot_ferret_config* cfg) {
+  if (!cfg || cfg->n <= 0 || cfg->k <= 0 || cfg->t <= 0 || cfg->h <= 0) return -1;
+  const int64_t M = cfg->k + (int64_t)cfg->h * cfg->t + 128;
+  return M < cfg->n ? state_bytes(M) : -1;
+}
+
+extern "C" int cuot_ferret_session_assemble_state(const cuot_ferret_session* s, int party, void* data, int64_t size) {
+  CUOT_CHECK_ARG(s && data, "NULL argument");
+  CUOT_CHECK_ARG(party == kAlice || party == kBob, "party must be 1 (ALICE, sender) or 2 (BOB, receiver)");
+  CUOT_CHECK_ARG(size >= state_bytes(s->M), "state buffer smaller than cuot_ferret_state_size");
+  unsigned char* a = (unsigned char*)data;
+  const int64_t head[4] = {(int64_t)party, s->plan.cfg.n, (int64_t)s->plan.cfg.t, s->plan.cfg.k};
+  memcpy(a, head, sizeof(head));
+  return export_party(s, party, (blk*)(a + sizeof(head)), a + sizeof(head) + 16);
+}
+
+extern "C" int cuot_ferret_session_disassemble_state(cuot_ctx* ctx, const cuot_ferret_config* cfg, int64_t round,
+                                                     const void* alice, const void* bob, int64_t size,
+                                                     cuot_ferret_session** out) {
+  CUOT_CHECK_ARG(ctx && cfg && alice && bob && out, "NULL argument");
+  *out = nullptr;
+  const int64_t need = cuot_ferret_state_size(cfg);
+  CUOT_CHECK_ARG(need > 0 && size >= need, "state buffers smaller than cuot_ferret_state_size");
+  const unsigned char* sa = (const unsigned char*)alice;
+  const unsigned char* sb = (const unsigned char*)bob;
+  int64_t ha[4], hb[4];
+  memcpy(ha, sa, sizeof(ha));
+  memcpy(hb, sb, sizeof(hb));
+  const bool match = ha[0] == kAlice && hb[0] == kBob && ha[1] == cfg->n && hb[1] == cfg->n && ha[2] == cfg->t &&
+                     hb[2] == cfg->t && ha[3] == cfg->k && hb[3] == cfg->k;
+  CUOT_CHECK_ARG(match, "state does not match the parties or the parameters (party, n, t, k)");
+  return import_session(ctx, cfg, round, sa + 32, sa + 48, sb + 48, out);
+}
+
+extern "C" int cuot_ferret_session_write_pre_file(const cuot_ferret_session* s, int party, const char* path) {
+  CUOT_CHECK_ARG(s && path, "NULL argument");
+  CUOT_CHECK_ARG(party == kAlice || party == kBob, "party must be 1 (ALICE, sender) or 2 (BOB, receiver)");
+  std::vector<unsigned char> boot((size_t)s->M * 16);
+  blk delta;
+  int rc = export_party(s, party, &delta, boot.data());
+  if (rc) return rc;
+  FILE* f = fopen(path, "wb");
+  CUOT_CHECK_ARG(f, "create a directory to store pre-OT data");
+  const int64_t p64 = party;
+  const int64_t nkt[3] = {s->plan.cfg.n, (int64_t)s->plan.cfg.t, s->plan.cfg.k};
+  bool ok = fwrite(&p64, sizeof(p64), 1, f) == 1;
+  if (party == kAlice) ok = ok && fwrite(&delta, 16, 1, f) == 1;
+  ok = ok && fwrite(nkt, sizeof(nkt), 1, f) == 1;
+  ok = ok && fwrite(boot.data(), 1, boot.size(), f) == boot.size();
+  ok = (fclose(f) == 0) && ok;
+  CUOT_CHECK_ARG(ok, "cannot write the pre-OT file");
+  return CUOT_OK;
+}
+
+extern "C" int cuot_ferret_session_read_pre_files(cuot_ctx* ctx, const cuot_ferret_config* cfg, int64_t round,
+                                                  const char* alice_path, const char* bob_path,
+                                                  cuot_ferret_session** out) {
+  CUOT_CHECK_ARG(ctx && cfg && alice_path && bob_path && out, "NULL argument");
+  *out = nullptr;
+  const int64_t need = cuot_ferret_state_size(cfg);
+  CUOT_CHECK_ARG(need > 0, "invalid parameters");
+  const int64_t M = (need - 48) / 16;
+  std::vector<unsigned char> boot[2];
+  blk delta;
+  const char* paths[2] = {alice_path, bob_path};
+  for (int r = 0; r < 2; r++) {
+    FILE* f = fopen(paths[r], "rb");
+    CUOT_CHECK_ARG(f, "cannot open the pre-OT file");
+    int64_t party = 0, nkt[3] = {0, 0, 0};
+    bool ok = read_exact(f, &party, sizeof(party));
+    const bool party_ok = ok && party == (r ? kBob : kAlice);
+    if (party_ok && r == 0) ok = read_exact(f, &delta, 16);
+    ok = ok && party_ok && read_exact(f, nkt, sizeof(nkt));
+    const bool params_ok = ok && nkt[0] == cfg->n && nkt[1] == cfg->t && nkt[2] == cfg->k;
+    boot[r].resize((size_t)M * 16);
+    ok = ok && params_ok && read_exact(f, boot[r].data(), boot[r].size());
+    fclose(f);
+    CUOT_CHECK_ARG(party_ok, "wrong party");
+    CUOT_CHECK_ARG(params_ok, "wrong parameters");
+    CUOT_CHECK_ARG(ok, "pre-OT file too short");
+  }
+  int rc = import_session(ctx, cfg, round, &delta, boot[0].data(), boot[1].data(), out);
+  if (rc) return rc;
+  // the files are consumed, as emp-ot's reader removes them
+  remove(alice_path);
+  remove(bob_path);
   return CUOT_OK;
 }
 
@@ -243,12 +401,34 @@ extern "C" int cuot_ferret_session_delta(const cuot_ferret_session* s, cuot_bloc
   return CUOT_OK;
 }
 
+extern "C" int cuot_ferret_session_extend_stages(cuot_ferret_session* s, cuot_block* out_s_d, cuot_block* out_r_d,
+                                                 int stages, void* stream) {
+  CUOT_CHECK_ARG(s && out_s_d && out_r_d, "NULL argument");
+  CUOT_CHECK_ARG(stages >= 1 && stages <= 3, "stages must be 1 (MPCOT), 2 (LPN) or 3 (both)");
+  if ((stages & 1) && s->seeded) {
+    cuot_set_last_error("the MPCOT stage already ran; run the LPN stage next", __FILE__, __LINE__);
+    return CUOT_ERR_STATE;
+  }
+  if (stages == 2 && !s->seeded) {
+    cuot_set_last_error("the LPN stage needs the MPCOT stage of the same extend first", __FILE__, __LINE__);
+    return CUOT_ERR_STATE;
+  }
+  CUOT_HIP_TRY(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->st(stream);
+  if (stages & 1) {
+    CUOT_HIP_TRY(extend_seed(s, (blk*)out_s_d, (blk*)out_r_d, st));
+    s->seeded = true;
+  }
+  if (stages & 2) {
+    CUOT_HIP_TRY(extend_lpn(s, (blk*)out_s_d, (blk*)out_r_d, st));
+    s->seeded = false;
+  }
+  return CUOT_OK;
+}
+
 extern "C" int cuot_ferret_session_extend(cuot_ferret_session* s, cuot_block* out_s_d, cuot_block* out_r_d,
                                           void* stream) {
-  CUOT_CHECK_ARG(s && out_s_d && out_r_d, "NULL argument");
-  CUOT_HIP_TRY(hipSetDevice(s->ctx->device));
-  CUOT_HIP_TRY(extend(s, (blk*)out_s_d, (blk*)out_r_d, s->ctx->st(stream)));
-  return CUOT_OK;
+  return cuot_ferret_session_extend_stages(s, out_s_d, out_r_d, 3, stream);
 }
 
 extern "C" int cuot_ferret_session_free(cuot_ferret_session* s) {

@@ -9,6 +9,13 @@ of the HIP library (cuot_ferret_session_*).
 from the current extend's ot_limit outputs and runs extend_f2k when they run
 out; `silent_ot_left` is the number still unserved.  The outputs are torch
 device tensors of shape (num, 2) int64 (one 16-byte block per row).
+
+Session state in emp-ot's byte layouts: `assemble_state` / `disassemble_state`
+/ `state_size` (ferret_cot.hpp:309-347) and `write_pre_data128_to_file` /
+`read_pre_data128_from_file` (ferret_cot.hpp:215-247), per party (ALICE = 1
+sender, BOB = 2 receiver).  With a `logger.Log` open for a role, each extend
+is timed as emp-ot logs it: SeedExp (pre-OT + MPCOT) and LPN
+(ferret_cot.hpp:84-92), for both roles since both run here.
 """
 from __future__ import annotations
 
@@ -16,10 +23,13 @@ import ctypes as C
 
 from . import _lib
 from ._lib import FerretConfig, check
+from .logger import LPN, Log, Recver, SeedExp, Sender
+
+ALICE, BOB = 1, 2  # emp-ot party ids
 
 
 class FerretCOT:
-    def __init__(self, ctx, n: int, k: int, t: int, h: int, seed, ntasks: int = 1):
+    def __init__(self, ctx, n: int, k: int, t: int, h: int, seed, ntasks: int = 1, _handle=None):
         import torch
 
         from . import to_block
@@ -27,8 +37,14 @@ class FerretCOT:
         self.ctx = ctx
         self.lib = _lib.load()
         cfg = FerretConfig(n=n, k=k, t=t, h=h, ntasks=ntasks, seed=to_block(seed))
-        hnd = C.c_void_p()
-        check(self.lib.cuot_ferret_session_new(ctx.handle, C.byref(cfg), C.byref(hnd)), "cuot_ferret_session_new")
+        self._cfg = cfg
+        self._params = dict(n=n, k=k, t=t, h=h, seed=seed, ntasks=ntasks)
+        if _handle is None:
+            hnd = C.c_void_p()
+            check(self.lib.cuot_ferret_session_new(ctx.handle, C.byref(cfg), C.byref(hnd)),
+                  "cuot_ferret_session_new")
+        else:
+            hnd = _handle
         self.handle = hnd
         M, lim, rnd = C.c_int64(), C.c_int64(), C.c_int64()
         check(self.lib.cuot_ferret_session_info(hnd, C.byref(M), C.byref(lim), C.byref(rnd)),
@@ -58,9 +74,64 @@ class FerretCOT:
 
     def extend_f2k(self):
         """One extend: n fresh COTs per party, the last M kept as the bootstrap."""
-        check(self.lib.cuot_ferret_session_extend(self.handle, self.out_s.data_ptr(), self.out_r.data_ptr(),
-                                                  self._stream()), "cuot_ferret_session_extend")
+        args = (self.handle, self.out_s.data_ptr(), self.out_r.data_ptr())
+        if not (Log.is_open(Sender) or Log.is_open(Recver)):
+            check(self.lib.cuot_ferret_session_extend(*args, self._stream()), "cuot_ferret_session_extend")
+        else:
+            import torch
+
+            for stage, event in ((1, SeedExp), (2, LPN)):
+                for role in (Sender, Recver):
+                    Log.start(role, event)
+                check(self.lib.cuot_ferret_session_extend_stages(*args, stage, self._stream()),
+                      "cuot_ferret_session_extend_stages")
+                torch.cuda.synchronize()
+                for role in (Sender, Recver):
+                    Log.mem(role, event)
+                    Log.end(role, event)
         self.ot_used = 0
+
+    # ---- state blob and pre-OT files (emp-ot layouts) ------------------------
+    def state_size(self) -> int:
+        return int(self.lib.cuot_ferret_state_size(C.byref(self._cfg)))
+
+    def assemble_state(self, party: int) -> bytes:
+        """One party's state blob: int64 party, n, t, k; Delta; bootstrap."""
+        buf = C.create_string_buffer(self.state_size())
+        check(self.lib.cuot_ferret_session_assemble_state(self.handle, party, buf, len(buf)),
+              "cuot_ferret_session_assemble_state")
+        return buf.raw
+
+    @classmethod
+    def disassemble_state(cls, ctx, alice: bytes, bob: bytes, n: int, k: int, t: int, h: int, seed,
+                          round: int = 0, ntasks: int = 1):
+        """A session resumed from both parties' state blobs at `round`."""
+        from . import to_block
+
+        cfg = FerretConfig(n=n, k=k, t=t, h=h, ntasks=ntasks, seed=to_block(seed))
+        hnd = C.c_void_p()
+        size = min(len(alice), len(bob))
+        check(_lib.load().cuot_ferret_session_disassemble_state(ctx.handle, C.byref(cfg), round, alice, bob, size,
+                                                                C.byref(hnd)),
+              "cuot_ferret_session_disassemble_state")
+        return cls(ctx, n, k, t, h, seed, ntasks, _handle=hnd)
+
+    def write_pre_data128_to_file(self, party: int, filename: str):
+        check(self.lib.cuot_ferret_session_write_pre_file(self.handle, party, filename.encode()),
+              "cuot_ferret_session_write_pre_file")
+
+    @classmethod
+    def read_pre_data128_from_file(cls, ctx, alice_file: str, bob_file: str, n: int, k: int, t: int, h: int, seed,
+                                   round: int = 0, ntasks: int = 1):
+        """A session from both parties' pre-OT files (which are then removed)."""
+        from . import to_block
+
+        cfg = FerretConfig(n=n, k=k, t=t, h=h, ntasks=ntasks, seed=to_block(seed))
+        hnd = C.c_void_p()
+        check(_lib.load().cuot_ferret_session_read_pre_files(ctx.handle, C.byref(cfg), round, alice_file.encode(),
+                                                             bob_file.encode(), C.byref(hnd)),
+              "cuot_ferret_session_read_pre_files")
+        return cls(ctx, n, k, t, h, seed, ntasks, _handle=hnd)
 
     def rcot(self, num: int):
         """num random COTs for both parties (sender blocks, receiver blocks)."""

@@ -14,6 +14,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import Block, SilentConfig as _SilentConfigC, check
+from .logger import LPN, BaseOT, Log, Recver, SeedExp, Sender
 
 
 @dataclass
@@ -54,6 +55,7 @@ device_bytes = _torch_view
 
 class _Role:
     _new = None
+    _log_role = Sender
 
     def __init__(self, ctx, config: SilentConfig, seed):
         self.ctx = ctx
@@ -65,14 +67,24 @@ class _Role:
                                            C.byref(Block(int(sb[0]), int(sb[1]))), C.byref(h)), self._new)
         self.handle = h
 
+    # each stage is one blocking call, logged as the reference's events
+    # (gpu/logger.h:7-9) when a logger.Log is open for this role
     def base_ot(self):
+        Log.start(self._log_role, BaseOT)
         check(self.lib.cuot_sot_base_ot(self.handle), "base_ot")
+        Log.end(self._log_role, BaseOT)
 
     def seed_expand(self):
+        Log.start(self._log_role, SeedExp)
         check(self.lib.cuot_sot_seed_expand(self.handle), "seed_expand")
+        Log.mem(self._log_role, SeedExp)
+        Log.end(self._log_role, SeedExp)
 
     def dual_lpn(self):
+        Log.start(self._log_role, LPN)
         check(self.lib.cuot_sot_dual_lpn(self.handle), "dual_lpn")
+        Log.mem(self._log_role, LPN)
+        Log.end(self._log_role, LPN)
 
     def _outputs(self):
         v, b, d, n = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_int64()
@@ -109,6 +121,7 @@ class SOTSender(_Role):
 class SOTRecver(_Role):
     """SOTRecver (silent/roles.h:66-85): puncVector (n blocks), choiceVector (n bits)."""
     _new = "cuot_sot_recver_new"
+    _log_role = Recver
 
     def get_punc_key(self, sender: SOTSender):
         check(self.lib.cuot_sot_get_punc_key(self.handle, sender.handle), "get_punc_key")

@@ -412,7 +412,35 @@ int cuot_ferret_session_info(const cuot_ferret_session *s, int64_t *M, int64_t *
 int cuot_ferret_session_delta(const cuot_ferret_session *s, cuot_block *delta_d, void *stream);
 /* out_s_d, out_r_d: n blocks each (device); out_r = out_s ^ lsb(out_r) * Delta */
 int cuot_ferret_session_extend(cuot_ferret_session *s, cuot_block *out_s_d, cuot_block *out_r_d, void *stream);
+/* one extend in two calls, for per-stage timing as emp-ot logs it
+ * (ferret_cot.hpp:84-92: SeedExp = pre-OT + MPCOT, LPN = LPN + bootstrap
+ * refresh): stages 1 then 2 on the same outputs (3 = both = _extend);
+ * out of order -> CUOT_ERR_STATE */
+int cuot_ferret_session_extend_stages(cuot_ferret_session *s, cuot_block *out_s_d, cuot_block *out_r_d, int stages,
+                                      void *stream);
 int cuot_ferret_session_free(cuot_ferret_session *s);
+
+/* ---- session state blob and pre-OT files (emp-ot byte layouts) --------------------
+ * party: 1 = ALICE (sender, holds Delta), 2 = BOB (receiver).  The bootstrap of
+ * M = k + h*t + 128 COTs plays emp-ot's ot_pre_data of n_pre blocks.
+ * State blob (FerretCOT::assemble_state / disassemble_state / state_size,
+ * ferret_cot.hpp:309-347): int64 party, n, t, k; 16-byte Delta (zero for BOB);
+ * M blocks.  The session stays usable after assembling.  Loading checks the
+ * party and (n, t, k) like disassemble_state (mismatch -> CUOT_ERR_ARG) and
+ * resumes at `round` (the roots of round r come from the seed's stream 1 at
+ * r*t, a counter the emp-ot blob does not carry).
+ * Pre-OT file (write_pre_data128_to_file / read_pre_data128_from_file,
+ * ferret_cot.hpp:215-247): int64 party; ALICE only: 16-byte Delta; int64 n,
+ * t, k; M blocks.  Reading checks party and parameters ("wrong party",
+ * "wrong parameters") and removes both files afterwards, as emp-ot does. */
+int64_t cuot_ferret_state_size(const cuot_ferret_config *cfg); /* -1 if invalid */
+int cuot_ferret_session_assemble_state(const cuot_ferret_session *s, int party, void *data, int64_t size);
+int cuot_ferret_session_disassemble_state(cuot_ctx *ctx, const cuot_ferret_config *cfg, int64_t round,
+                                          const void *alice, const void *bob, int64_t size,
+                                          cuot_ferret_session **out);
+int cuot_ferret_session_write_pre_file(const cuot_ferret_session *s, int party, const char *path);
+int cuot_ferret_session_read_pre_files(cuot_ctx *ctx, const cuot_ferret_config *cfg, int64_t round,
+                                       const char *alice_path, const char *bob_path, cuot_ferret_session **out);
 
 #ifdef __cplusplus
 }

@@ -321,3 +321,32 @@ def ccrh(x):
     out = blocks(x.shape[0])
     lib().or_ccrh(_p(x), _p(out), x.shape[0])
     return out
+
+
+# ---- emp-ot Ferret session byte layouts --------------------------------------
+# (ferret/emp-ot/emp-ot/ferret/ferret_cot.hpp; party 1 = ALICE, 2 = BOB)
+def ferret_state_blob(party, n, t, k, delta, boot):
+    """FerretCOT::assemble_state (ferret_cot.hpp:309-322): int64 party at 0,
+    n at 8, t at 16, k at 24, the 16-byte Delta at 32, ot_pre_data from 48."""
+    head = np.array([party, n, t, k], dtype="<i8").tobytes()
+    d = np.ascontiguousarray(delta, dtype="<u8").reshape(2).tobytes()
+    return head + d + np.ascontiguousarray(boot, dtype="<u8").tobytes()
+
+
+def ferret_state_parse(blob, n_pre):
+    """FerretCOT::disassemble_state (ferret_cot.hpp:324-342): the header fields,
+    Delta and n_pre blocks."""
+    head = np.frombuffer(blob[:32], dtype="<i8")
+    delta = np.frombuffer(blob[32:48], dtype="<u8")
+    boot = np.frombuffer(blob[48:48 + 16 * n_pre], dtype="<u8").reshape(n_pre, 2)
+    return [int(v) for v in head], delta, boot
+
+
+def ferret_pre_file(party, n, t, k, delta, boot):
+    """write_pre_data128_to_file (ferret_cot.hpp:215-227): int64 party; the
+    16-byte delta for ALICE only; int64 n, t, k; n_pre blocks."""
+    out = np.array([party], dtype="<i8").tobytes()
+    if party == 1:
+        out += np.ascontiguousarray(delta, dtype="<u8").reshape(2).tobytes()
+    out += np.array([n, t, k], dtype="<i8").tobytes()
+    return out + np.ascontiguousarray(boot, dtype="<u8").tobytes()

@@ -131,6 +131,8 @@ SIGNATURES = {
     "cuot_ferret_session_extend": (INT, [P, P, P, P]),
     "cuot_ferret_session_free": (INT, [P]),
     "cuot_ferret_session_extend_stages": (INT, [P, P, P, INT, P]),
+    "cuot_ferret_session_set_malicious": (INT, [P, INT]),
+    "cuot_ferret_session_check": (INT, [P, C.POINTER(INT), P, P, P]),
     "cuot_ferret_state_size": (I64, [C.POINTER(FerretConfig)]),
     "cuot_ferret_session_assemble_state": (INT, [P, INT, P, I64]),
     "cuot_ferret_session_disassemble_state": (INT, [P, C.POINTER(FerretConfig), I64, P, P, I64, C.POINTER(P)]),

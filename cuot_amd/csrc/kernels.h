@@ -137,5 +137,12 @@ hipError_t cuot_launch_ferret_sums(const blk* sums, const blk* delta, const uint
 hipError_t cuot_launch_ferret_boot(blk* K, blk* Mr, const uint8_t* cbits, const blk* delta, int64_t k,
                                    hipStream_t st);
 
+// Malicious-security consistency check of a Ferret MPCOT (consist.hip):
+// per-tree V, W, chi_alpha into vwc (3 t blocks), then both parties' check
+// into out (x' 128 bytes, ALICE digest 32, BOB digest 32, ok int32).
+hipError_t cuot_launch_consist(const blk* leaves_s, const blk* leaves_r, int64_t n, int64_t t, int h,
+                               const blk* ssum, const uint8_t* bits, const blk* pre_s, const blk* pre_r,
+                               const blk* delta, blk* vwc, uint8_t* out, hipStream_t st);
+
 // Host single-block AES-128 (aes.hip), rk from cuot_aes_expand_host.
 void cuot_aes_encrypt_host(const uint32_t rk[44], const uint8_t in[16], uint8_t out[16]);

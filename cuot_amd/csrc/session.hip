@@ -91,6 +91,9 @@ struct cuot_ferret_session {
   FerretPlan plan;  // parameters, GGM/LPN argument builders
   int64_t npre = 0, M = 0, ot_limit = 0, round = 0;
   bool seeded = false;  // the MPCOT half of the current extend has run
+  bool malicious = false;  // run the consistency check after each MPCOT
+  blk* vwc = nullptr;      // per-tree V, W, chi_alpha (3 t blocks)
+  uint8_t* chk = nullptr;  // x' (128), digests (2 x 32), ok (int32)
   char* mem = nullptr;
   blk *delta, *boot_s, *boot_r, *P0, *P1, *Pr, *roots, *sums, *csum, *ssum, *cbits, *kkp;
   uint8_t* bits;
@@ -120,7 +123,17 @@ size_t carve_session(cuot_ferret_session* s, char* base) {
   s->ssum = (blk*)take((size_t)t * 16);
   s->cbits = (blk*)take((size_t)((s->M + 127) / 128) * 16);
   s->kkp = (blk*)take((size_t)2 * s->plan.cfg.k * 16);
+  s->vwc = (blk*)take((size_t)3 * t * 16);
+  s->chk = (uint8_t*)take(256);
   return off + 256;
+}
+
+// the f2k consistency check on the current trees' leaves and the first 128
+// bootstrap COTs (mpcot_reg.h:94-95: pre_cot_data = ot_input)
+hipError_t consistency(cuot_ferret_session* s, const blk* out_s, const blk* out_r, hipStream_t st) {
+  const cuot_ferret_config& cfg = s->plan.cfg;
+  return cuot_launch_consist(out_s, out_r, cfg.n, cfg.t, cfg.h, s->ssum, s->bits, s->boot_s, s->boot_r, s->delta,
+                             s->vwc, s->chk, st);
 }
 
 // the first half of an extend (emp-ot's SeedExp event): pre-OT and MPCOT
@@ -168,7 +181,9 @@ hipError_t extend_seed(cuot_ferret_session* s, blk* out_s, blk* out_r, hipStream
   r.choice = s->bits;
   r.csum = s->csum;
   r.fin = s->ssum;
-  return cuot_launch_ggm(r, true, st);
+  if ((e = cuot_launch_ggm(r, true, st))) return e;
+  if (s->malicious) return consistency(s, out_s, out_r, st);
+  return hipSuccess;
 }
 
 // the second half (emp-ot's LPN event): LPN and the bootstrap refresh
@@ -404,7 +419,17 @@ extern "C" int cuot_ferret_session_delta(const cuot_ferret_session* s, cuot_bloc
 extern "C" int cuot_ferret_session_extend_stages(cuot_ferret_session* s, cuot_block* out_s_d, cuot_block* out_r_d,
                                                  int stages, void* stream) {
   CUOT_CHECK_ARG(s && out_s_d && out_r_d, "NULL argument");
-  CUOT_CHECK_ARG(stages >= 1 && stages <= 3, "stages must be 1 (MPCOT), 2 (LPN) or 3 (both)");
+  CUOT_CHECK_ARG(stages >= 1 && stages <= 4, "stages must be 1 (MPCOT), 2 (LPN), 3 (both) or 4 (check)");
+  if (stages == 4) {  // the consistency check alone, between the two halves
+    if (!s->seeded) {
+      cuot_set_last_error("the consistency check needs the MPCOT stage of the same extend first", __FILE__,
+                          __LINE__);
+      return CUOT_ERR_STATE;
+    }
+    CUOT_HIP_TRY(hipSetDevice(s->ctx->device));
+    CUOT_HIP_TRY(consistency(s, (const blk*)out_s_d, (const blk*)out_r_d, s->ctx->st(stream)));
+    return CUOT_OK;
+  }
   if ((stages & 1) && s->seeded) {
     cuot_set_last_error("the MPCOT stage already ran; run the LPN stage next", __FILE__, __LINE__);
     return CUOT_ERR_STATE;
@@ -423,6 +448,28 @@ extern "C" int cuot_ferret_session_extend_stages(cuot_ferret_session* s, cuot_bl
     CUOT_HIP_TRY(extend_lpn(s, (blk*)out_s_d, (blk*)out_r_d, st));
     s->seeded = false;
   }
+  return CUOT_OK;
+}
+
+extern "C" int cuot_ferret_session_set_malicious(cuot_ferret_session* s, int on) {
+  CUOT_CHECK_ARG(s, "NULL session");
+  s->malicious = on != 0;
+  return CUOT_OK;
+}
+
+extern "C" int cuot_ferret_session_check(const cuot_ferret_session* s, int* ok, uint8_t* xprime, uint8_t* dig_a,
+                                         uint8_t* dig_b) {
+  CUOT_CHECK_ARG(s && ok, "NULL argument");
+  uint8_t h[196];
+  CUOT_HIP_TRY(hipSetDevice(s->ctx->device));
+  CUOT_HIP_TRY(hipDeviceSynchronize());
+  CUOT_HIP_TRY(hipMemcpy(h, s->chk, sizeof(h), hipMemcpyDeviceToHost));
+  int32_t v;
+  memcpy(&v, h + 192, 4);
+  *ok = v;
+  if (xprime) memcpy(xprime, h, 128);
+  if (dig_a) memcpy(dig_a, h + 128, 32);
+  if (dig_b) memcpy(dig_b, h + 160, 32);
   return CUOT_OK;
 }
 

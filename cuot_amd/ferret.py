@@ -91,6 +91,19 @@ class FerretCOT:
                     Log.end(role, event)
         self.ot_used = 0
 
+    # ---- malicious mode ----------------------------------------------------
+    def set_malicious(self, on: bool = True):
+        """MpcotReg::set_malicious: check every MPCOT (both parties)."""
+        check(self.lib.cuot_ferret_session_set_malicious(self.handle, 1 if on else 0),
+              "cuot_ferret_session_set_malicious")
+
+    def check_result(self):
+        """The last consistency check: (ok, x' bytes, ALICE digest, BOB digest)."""
+        ok = C.c_int()
+        xp, da, db = (C.c_uint8 * 128)(), (C.c_uint8 * 32)(), (C.c_uint8 * 32)()
+        check(self.lib.cuot_ferret_session_check(self.handle, C.byref(ok), xp, da, db), "cuot_ferret_session_check")
+        return bool(ok.value), bytes(xp), bytes(da), bytes(db)
+
     # ---- state blob and pre-OT files (emp-ot layouts) ------------------------
     def state_size(self) -> int:
         return int(self.lib.cuot_ferret_state_size(C.byref(self._cfg)))

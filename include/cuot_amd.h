@@ -415,10 +415,23 @@ int cuot_ferret_session_extend(cuot_ferret_session *s, cuot_block *out_s_d, cuot
 /* one extend in two calls, for per-stage timing as emp-ot logs it
  * (ferret_cot.hpp:84-92: SeedExp = pre-OT + MPCOT, LPN = LPN + bootstrap
  * refresh): stages 1 then 2 on the same outputs (3 = both = _extend);
- * out of order -> CUOT_ERR_STATE */
+ * stage 4 (between 1 and 2) runs the consistency check alone on the current
+ * outputs; out of order -> CUOT_ERR_STATE */
 int cuot_ferret_session_extend_stages(cuot_ferret_session *s, cuot_block *out_s_d, cuot_block *out_r_d, int stages,
                                       void *stream);
 int cuot_ferret_session_free(cuot_ferret_session *s);
+/* Malicious mode (MpcotReg::set_malicious, mpcot_reg.h:68-70): every MPCOT is
+ * followed by the f2k consistency check of both parties (spcot_sender.h:88-98,
+ * spcot_recver.h:97-107, mpcot_reg.h:177-220) with emp-tool's arithmetic
+ * (SHA-256 seeds, chi_j = seed^(j+1) and inner products in GF(2^128) mod
+ * x^128 + x^7 + x^2 + x + 1, packing of the first 128 bootstrap COTs).
+ * Leaves not stored (global index >= n) are left out, and so is the
+ * punctured leaf's chi when that leaf is not stored. */
+int cuot_ferret_session_set_malicious(cuot_ferret_session *s, int on);
+/* the last check (waits for the device): ok = 1 when ALICE's and BOB's
+ * digests agree; x' (128 bytes, BOB's message), digests (32 bytes each) */
+int cuot_ferret_session_check(const cuot_ferret_session *s, int *ok, uint8_t *xprime, uint8_t *dig_a,
+                              uint8_t *dig_b);
 
 /* ---- session state blob and pre-OT files (emp-ot byte layouts) --------------------
  * party: 1 = ALICE (sender, holds Delta), 2 = BOB (receiver).  The bootstrap of

@@ -849,6 +849,18 @@ void or_ferret_session_extend(or_ferret_session *s, int nthreads, orb *out_s, or
     csum[i] = bx(bits[i] ? pad1 : pad0, Pr[i]);
   }
   or_ferret_mpcot_recver(t, h, bits, csum, ss, n, out_r, nthreads);
+  /* malicious mode (MpcotReg::set_malicious): the f2k consistency check on
+   * the trees' leaves and the first 128 bootstrap COTs */
+  if (s->malicious) {
+    orb *V = (orb *)malloc((size_t)t * sizeof(orb));
+    orb *Wt = (orb *)malloc((size_t)t * sizeof(orb));
+    orb *ca = (orb *)malloc((size_t)t * sizeof(orb));
+    or_ferret_consist_trees(out_s, out_r, n, t, h, ss, bits, V, Wt, ca, nthreads);
+    s->check_ok = or_ferret_consist_final(V, Wt, ca, t, s->boot_s, s->boot_r, D, s->xprime, s->dig_a, s->dig_b);
+    free(V);
+    free(Wt);
+    free(ca);
+  }
   /* LPN with kk = bootstrap + 128 */
   orb seed_lpn;
   or_prg(&cfg->seed, 6, 0, 1, &seed_lpn);
@@ -874,4 +886,174 @@ void or_ferret_session_free(or_ferret_session *s) {
   free(s->boot_s);
   free(s->boot_r);
   s->boot_s = s->boot_r = NULL;
+}
+
+
+/* ==== malicious consistency check (emp-ot's f2k check; emp-tool arithmetic) ====
+ * emp-tool @44b1dde is absent: its Hash (SHA-256), gfmul, uni_hash_coeff_gen,
+ * vector_inn_prdt_sum_red and GaloisFieldPacking are restated from their
+ * published definitions and the call sites spcot_sender.h:88-98,
+ * spcot_recver.h:97-107, mpcot_reg.h:177-220. */
+
+/* SHA-256, FIPS 180-4 section 6.2 */
+static const uint32_t SHA_K[64] = {
+    0x428a2f98u, 0x71374491u, 0xb5c0fbcfu, 0xe9b5dba5u, 0x3956c25bu, 0x59f111f1u, 0x923f82a4u, 0xab1c5ed5u,
+    0xd807aa98u, 0x12835b01u, 0x243185beu, 0x550c7dc3u, 0x72be5d74u, 0x80deb1feu, 0x9bdc06a7u, 0xc19bf174u,
+    0xe49b69c1u, 0xefbe4786u, 0x0fc19dc6u, 0x240ca1ccu, 0x2de92c6fu, 0x4a7484aau, 0x5cb0a9dcu, 0x76f988dau,
+    0x983e5152u, 0xa831c66du, 0xb00327c8u, 0xbf597fc7u, 0xc6e00bf3u, 0xd5a79147u, 0x06ca6351u, 0x14292967u,
+    0x27b70a85u, 0x2e1b2138u, 0x4d2c6dfcu, 0x53380d13u, 0x650a7354u, 0x766a0abbu, 0x81c2c92eu, 0x92722c85u,
+    0xa2bfe8a1u, 0xa81a664bu, 0xc24b8b70u, 0xc76c51a3u, 0xd192e819u, 0xd6990624u, 0xf40e3585u, 0x106aa070u,
+    0x19a4c116u, 0x1e376c08u, 0x2748774cu, 0x34b0bcb5u, 0x391c0cb3u, 0x4ed8aa4au, 0x5b9cca4fu, 0x682e6ff3u,
+    0x748f82eeu, 0x78a5636fu, 0x84c87814u, 0x8cc70208u, 0x90befffau, 0xa4506cebu, 0xbef9a3f7u, 0xc67178f2u};
+
+static inline uint32_t ror32(uint32_t x, int r) { return (x >> r) | (x << (32 - r)); }
+
+static void sha256_block(uint32_t H[8], const uint8_t blk[64]) {
+  uint32_t w[64];
+  for (int i = 0; i < 16; i++)
+    w[i] = (uint32_t)blk[4 * i] << 24 | (uint32_t)blk[4 * i + 1] << 16 | (uint32_t)blk[4 * i + 2] << 8 | blk[4 * i + 3];
+  for (int i = 16; i < 64; i++) {
+    const uint32_t s0 = ror32(w[i - 15], 7) ^ ror32(w[i - 15], 18) ^ (w[i - 15] >> 3);
+    const uint32_t s1 = ror32(w[i - 2], 17) ^ ror32(w[i - 2], 19) ^ (w[i - 2] >> 10);
+    w[i] = w[i - 16] + s0 + w[i - 7] + s1;
+  }
+  uint32_t a = H[0], b = H[1], c = H[2], d = H[3], e = H[4], f = H[5], g = H[6], h = H[7];
+  for (int i = 0; i < 64; i++) {
+    const uint32_t S1 = ror32(e, 6) ^ ror32(e, 11) ^ ror32(e, 25);
+    const uint32_t ch = (e & f) ^ (~e & g);
+    const uint32_t t1 = h + S1 + ch + SHA_K[i] + w[i];
+    const uint32_t S0 = ror32(a, 2) ^ ror32(a, 13) ^ ror32(a, 22);
+    const uint32_t mj = (a & b) ^ (a & c) ^ (b & c);
+    const uint32_t t2 = S0 + mj;
+    h = g;
+    g = f;
+    f = e;
+    e = d + t1;
+    d = c;
+    c = b;
+    b = a;
+    a = t1 + t2;
+  }
+  H[0] += a; H[1] += b; H[2] += c; H[3] += d; H[4] += e; H[5] += f; H[6] += g; H[7] += h;
+}
+
+void or_sha256(const uint8_t *msg, size_t len, uint8_t out[32]) {
+  uint32_t H[8] = {0x6a09e667u, 0xbb67ae85u, 0x3c6ef372u, 0xa54ff53au,
+                   0x510e527fu, 0x9b05688cu, 0x1f83d9abu, 0x5be0cd19u};
+  uint8_t blk[64];
+  size_t off = 0;
+  while (len - off >= 64) {
+    sha256_block(H, msg + off);
+    off += 64;
+  }
+  const size_t rem = len - off;
+  memset(blk, 0, 64);
+  memcpy(blk, msg + off, rem);
+  blk[rem] = 0x80;
+  if (rem >= 56) {
+    sha256_block(H, blk);
+    memset(blk, 0, 64);
+  }
+  const uint64_t bits = (uint64_t)len * 8;
+  for (int i = 0; i < 8; i++) blk[63 - i] = (uint8_t)(bits >> (8 * i));
+  sha256_block(H, blk);
+  for (int i = 0; i < 8; i++) {
+    out[4 * i] = (uint8_t)(H[i] >> 24);
+    out[4 * i + 1] = (uint8_t)(H[i] >> 16);
+    out[4 * i + 2] = (uint8_t)(H[i] >> 8);
+    out[4 * i + 3] = (uint8_t)H[i];
+  }
+}
+
+/* GF(2^128) = GF(2)[x]/(x^128 + x^7 + x^2 + x + 1); bit i of the block
+ * (lo bit 0 = byte 0 bit 0) is the coefficient of x^i (emp-tool gfmul). */
+orb or_gf128_mul(orb a, orb b) {
+  uint64_t p[4] = {0, 0, 0, 0}; /* 256-bit carry-less product */
+  uint64_t l, h;
+  clmul_soft(a.lo, b.lo, &l, &h);
+  p[0] ^= l; p[1] ^= h;
+  clmul_soft(a.lo, b.hi, &l, &h);
+  p[1] ^= l; p[2] ^= h;
+  clmul_soft(a.hi, b.lo, &l, &h);
+  p[1] ^= l; p[2] ^= h;
+  clmul_soft(a.hi, b.hi, &l, &h);
+  p[2] ^= l; p[3] ^= h;
+  /* x^128 = x^7 + x^2 + x + 1: fold the high half twice */
+  for (int r = 0; r < 2; r++) {
+    const uint64_t h0 = p[2], h1 = p[3];
+    p[2] = p[3] = 0;
+    /* H * (1 + x + x^2 + x^7), 135 bits: low 128 into p[0..1], the rest into p[2] */
+    p[0] ^= h0 ^ (h0 << 1) ^ (h0 << 2) ^ (h0 << 7);
+    p[1] ^= h1 ^ (h1 << 1) ^ (h1 << 2) ^ (h1 << 7) ^ (h0 >> 63) ^ (h0 >> 62) ^ (h0 >> 57);
+    p[2] ^= (h1 >> 63) ^ (h1 >> 62) ^ (h1 >> 57);
+  }
+  orb o = {p[0], p[1]};
+  return o;
+}
+
+static orb gf128_pow(orb s, uint64_t e) {
+  orb r = {1, 0};
+  while (e) {
+    if (e & 1) r = or_gf128_mul(r, s);
+    s = or_gf128_mul(s, s);
+    e >>= 1;
+  }
+  return r;
+}
+
+static orb hash_seed(orb x) {
+  uint8_t d[32];
+  or_sha256((const uint8_t *)&x, 16, d);
+  orb s;
+  memcpy(&s, d, 16);
+  return s;
+}
+
+void or_ferret_consist_trees(const orb *leaves_s, const orb *leaves_r, int64_t n, int64_t t, int h,
+                             const orb *ssum, const uint8_t *bits, orb *V, orb *W, orb *chi_alpha, int nthreads) {
+  const int64_t L = (int64_t)1 << h;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic)
+  for (int64_t tr = 0; tr < t; tr++) {
+    const orb s = hash_seed(ssum[tr]);
+    int64_t alpha = 0;
+    for (int d = 0; d < h; d++) alpha = (alpha << 1) + (bits[tr * h + d] ? 0 : 1);
+    orb chi = s, v = {0, 0}, w = {0, 0};
+    for (int64_t j = 0; j < L; j++) {
+      const int64_t g = tr * L + j;
+      if (g >= n) break;
+      v = bx(v, or_gf128_mul(chi, leaves_s[g]));
+      w = bx(w, or_gf128_mul(chi, leaves_r[g]));
+      chi = or_gf128_mul(chi, s);
+    }
+    V[tr] = v;
+    W[tr] = w;
+    const orb zero = {0, 0};
+    chi_alpha[tr] = (tr * L + alpha < n) ? gf128_pow(s, (uint64_t)alpha + 1) : zero;
+  }
+}
+
+int or_ferret_consist_final(const orb *V, const orb *W, const orb *chi_alpha, int64_t t, const orb *pre_s,
+                            const orb *pre_r, orb delta, uint8_t xprime[128], uint8_t dig_a[32],
+                            uint8_t dig_b[32]) {
+  orb r1a = {0, 0}, r1b = {0, 0}, r2 = {0, 0};
+  for (int64_t i = 0; i < t; i++) {
+    r1a = bx(r1a, V[i]);
+    r1b = bx(r1b, W[i]);
+    r2 = bx(r2, chi_alpha[i]);
+  }
+  orb pa = {0, 0}, pb = {0, 0};
+  for (int j = 0; j < 128; j++) {
+    const int rb = j < 64 ? (int)((r2.lo >> j) & 1) : (int)((r2.hi >> (j - 64)) & 1);
+    xprime[j] = (uint8_t)(rb ^ (int)(pre_r[j].lo & 1));
+    orb a = pre_s[j];
+    if (xprime[j]) a = bx(a, delta);
+    orb xj = {j < 64 ? 1ull << j : 0, j >= 64 ? 1ull << (j - 64) : 0};
+    pa = bx(pa, or_gf128_mul(a, xj));
+    pb = bx(pb, or_gf128_mul(pre_r[j], xj));
+  }
+  r1a = bx(r1a, pa);
+  r1b = bx(r1b, pb);
+  or_sha256((const uint8_t *)&r1a, 16, dig_a);
+  or_sha256((const uint8_t *)&r1b, 16, dig_b);
+  return memcmp(dig_a, dig_b, 32) == 0;
 }

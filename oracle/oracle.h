@@ -164,6 +164,9 @@ typedef struct {
   int64_t M, ot_limit, round;
   orb delta;
   orb *boot_s, *boot_r; /* M blocks each */
+  /* malicious mode: the last extend's consistency check */
+  int malicious, check_ok;
+  uint8_t xprime[128], dig_a[32], dig_b[32];
 } or_ferret_session;
 
 int or_ferret_session_init(or_ferret_session *s, const or_ferret_cfg *cfg);
@@ -172,6 +175,28 @@ int or_ferret_session_init(or_ferret_session *s, const or_ferret_cfg *cfg);
 void or_ferret_session_extend(or_ferret_session *s, int nthreads, orb *out_s, orb *out_r);
 void or_ferret_session_free(or_ferret_session *s);
 void or_ccrh(const orb *in, orb *out, size_t n);
+
+/* ---- malicious consistency check (mpcot_reg.h:177-220) ---------------------
+ * emp-tool's Hash, gfmul, uni_hash_coeff_gen, vector_inn_prdt_sum_red and
+ * GaloisFieldPacking restated (emp-tool @44b1dde is absent).
+ * GF(2^128) = GF(2)[x]/(x^128 + x^7 + x^2 + x + 1), bit i = coefficient x^i.
+ * Per tree: seed = SHA-256(secret_sum)[0:16] (spcot_sender.h:91-92),
+ * chi_j = seed^(j+1), V = sum chi_j leaf_s[j], W = sum chi_j leaf_r[j],
+ * chi_alpha = chi at the punctured leaf (spcot_recver.h:104).  Leaves at a
+ * global index >= n are not stored and are left out, and so is a chi_alpha
+ * whose leaf is not stored (emp-ot's own parameters store every leaf).
+ * bits: the receiver's choice bits [tree * h + level]. */
+void or_sha256(const uint8_t *msg, size_t len, uint8_t out[32]);
+orb or_gf128_mul(orb a, orb b);
+void or_ferret_consist_trees(const orb *leaves_s, const orb *leaves_r, int64_t n, int64_t t, int h,
+                             const orb *ssum, const uint8_t *bits, orb *V, orb *W, orb *chi_alpha, int nthreads);
+/* consistency_check_f2k for both parties: x' = bits of XOR chi_alpha ^
+ * LSB(pre_r); digests of XOR V ^ packing(pre_s ^ x' Delta) (ALICE) and
+ * XOR W ^ packing(pre_r) (BOB), packing(d) = sum_j d_j x^j; returns 1 when
+ * they agree */
+int or_ferret_consist_final(const orb *V, const orb *W, const orb *chi_alpha, int64_t t, const orb *pre_s,
+                            const orb *pre_r, orb delta, uint8_t xprime[128], uint8_t dig_a[32],
+                            uint8_t dig_b[32]);
 
 #ifdef __cplusplus
 }

@@ -38,7 +38,8 @@ class FERRETCFG(C.Structure):
 
 class FERRETSESSION(C.Structure):
     _fields_ = [("cfg", FERRETCFG), ("M", I64), ("ot_limit", I64), ("round", I64), ("delta", ORB),
-                ("boot_s", P), ("boot_r", P)]
+                ("boot_s", P), ("boot_r", P), ("malicious", C.c_int), ("check_ok", C.c_int),
+                ("xprime", C.c_uint8 * 128), ("dig_a", C.c_uint8 * 32), ("dig_b", C.c_uint8 * 32)]
 
 
 def build(force: bool = False) -> str:
@@ -300,6 +301,14 @@ class FerretSession:
         br = np.ctypeslib.as_array(C.cast(self.s.boot_r, C.POINTER(C.c_uint64)), shape=(self.s.M, 2)).copy()
         return bs, br
 
+    def set_malicious(self, on=True):
+        self.s.malicious = 1 if on else 0
+
+    def check_result(self):
+        """The last extend's consistency check: (ok, x', digest ALICE, digest BOB)."""
+        return (bool(self.s.check_ok), np.frombuffer(bytes(self.s.xprime), dtype=np.uint8).copy(),
+                bytes(self.s.dig_a), bytes(self.s.dig_b))
+
     def extend(self):
         o_s, o_r = blocks(self.n), blocks(self.n)
         lib().or_ferret_session_extend(C.byref(self.s), self.nthreads, _p(o_s), _p(o_r))
@@ -350,3 +359,42 @@ def ferret_pre_file(party, n, t, k, delta, boot):
         out += np.ascontiguousarray(delta, dtype="<u8").reshape(2).tobytes()
     out += np.array([n, t, k], dtype="<i8").tobytes()
     return out + np.ascontiguousarray(boot, dtype="<u8").tobytes()
+
+
+# ---- malicious consistency check (emp-tool arithmetic restated) ---------------
+def sha256(msg: bytes) -> bytes:
+    out = (C.c_uint8 * 32)()
+    buf = C.create_string_buffer(bytes(msg), len(msg))
+    lib().or_sha256(buf, C.c_size_t(len(msg)), out)
+    return bytes(out)
+
+
+def gf128_mul(a, b):
+    L = lib()
+    L.or_gf128_mul.restype = ORB
+    L.or_gf128_mul.argtypes = [ORB, ORB]
+    r = L.or_gf128_mul(as_orb(a), as_orb(b))
+    return np.array([r.lo, r.hi], dtype=np.uint64)
+
+
+def consist_trees(leaves_s, leaves_r, n, t, h, ssum, bits, nthreads=8):
+    """Per tree (V, W, chi_alpha) of the f2k consistency check."""
+    ls = np.ascontiguousarray(leaves_s, dtype=np.uint64).reshape(-1, 2)
+    lr = np.ascontiguousarray(leaves_r, dtype=np.uint64).reshape(-1, 2)
+    ss = np.ascontiguousarray(ssum, dtype=np.uint64).reshape(-1, 2)
+    bb = np.ascontiguousarray(bits, dtype=np.uint8)
+    V, W, ca = blocks(t), blocks(t), blocks(t)
+    lib().or_ferret_consist_trees(_p(ls), _p(lr), C.c_int64(n), C.c_int64(t), C.c_int(h), _p(ss), _p(bb), _p(V),
+                                  _p(W), _p(ca), C.c_int(nthreads))
+    return V, W, ca
+
+
+def consist_final(V, W, chi_alpha, pre_s, pre_r, delta):
+    """consistency_check_f2k for both parties: (ok, x', digest ALICE, digest BOB)."""
+    V, W, ca = (np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, 2) for x in (V, W, chi_alpha))
+    ps = np.ascontiguousarray(pre_s, dtype=np.uint64).reshape(-1, 2)
+    pr = np.ascontiguousarray(pre_r, dtype=np.uint64).reshape(-1, 2)
+    xp, da, db = (C.c_uint8 * 128)(), (C.c_uint8 * 32)(), (C.c_uint8 * 32)()
+    ok = lib().or_ferret_consist_final(_p(V), _p(W), _p(ca), C.c_int64(V.shape[0]), _p(ps), _p(pr), as_orb(delta),
+                                       xp, da, db)
+    return bool(ok), np.frombuffer(bytes(xp), dtype=np.uint8).copy(), bytes(da), bytes(db)

@@ -132,6 +132,9 @@ SIGNATURES = {
     "cuot_ferret_session_free": (INT, [P]),
     "cuot_ferret_session_extend_stages": (INT, [P, P, P, INT, P]),
     "cuot_ferret_session_set_malicious": (INT, [P, INT]),
+    "cuot_cot_online": (INT, [P, P, P, P, P, P, I64, P]),
+    "cuot_cot_to_ot": (INT, [P, P, P, P, Block, C.c_uint64, P, P, P, P, P, I64, P]),
+    "cuot_cot_to_rot": (INT, [P, P, P, P, Block, C.c_uint64, P, P, P, I64, P]),
     "cuot_ferret_session_check": (INT, [P, C.POINTER(INT), P, P, P]),
     "cuot_ferret_state_size": (I64, [C.POINTER(FerretConfig)]),
     "cuot_ferret_session_assemble_state": (INT, [P, INT, P, I64]),

@@ -144,5 +144,14 @@ hipError_t cuot_launch_consist(const blk* leaves_s, const blk* leaves_r, int64_t
                                const blk* ssum, const uint8_t* bits, const blk* pre_s, const blk* pre_r,
                                const blk* delta, blk* vwc, uint8_t* out, hipStream_t st);
 
+// Online derandomisation (derand.hip): chosen-bit COT in place, and the
+// MITCCRH hash pass turning COTs into OTs (pad, out_r) or random OTs (rot0,
+// rot1 non-NULL; out_r = H(M)).
+hipError_t cuot_launch_cot_online(blk* ks, const blk* mr, const blk* delta, const uint8_t* b, uint8_t* bo, int64_t n,
+                                  hipStream_t st);
+hipError_t cuot_launch_cot_hash(const blk* ks, const blk* mr, const blk* delta, blk s, uint64_t g0, int64_t n,
+                                const uint8_t* r, const blk* m0, const blk* m1, blk* pad, blk* out_r, blk* rot0,
+                                blk* rot1, hipStream_t st);
+
 // Host single-block AES-128 (aes.hip), rk from cuot_aes_expand_host.
 void cuot_aes_encrypt_host(const uint32_t rk[44], const uint8_t in[16], uint8_t out[16]);

@@ -91,6 +91,52 @@ class FerretCOT:
                     Log.end(role, event)
         self.ot_used = 0
 
+    # ---- online derandomisation (both parties) --------------------------------
+    def send_recv_cot(self, b):
+        """FerretCOT::send_cot + recv_cot (ferret_cot.hpp:297-307): len(b) COTs
+        with the receiver's chosen bits b (uint8 device tensor, one per COT):
+        (sender K, receiver M = K ^ b*Delta, the receiver's message bo)."""
+        import torch
+
+        num = int(b.shape[0])
+        ks, mr = self.rcot(num)
+        bo = torch.empty(num, dtype=torch.uint8, device=ks.device)
+        check(self.lib.cuot_cot_online(self.ctx.handle, ks.data_ptr(), mr.data_ptr(), b.data_ptr(),
+                                       self.Delta.data_ptr(), bo.data_ptr(), num, self._stream()), "cuot_cot_online")
+        return ks, mr, bo
+
+    def send_recv_ot(self, m0, m1, r, s, key_start: int = 0):
+        """COT::send / recv (cot.h:16-56): chosen-message OTs, the receiver
+        learning m_r; s = the sender's MITCCRH block.  (sender pads, receiver
+        outputs)."""
+        import torch
+
+        from . import to_block
+
+        ks, mr, _ = self.send_recv_cot(r)
+        num = int(r.shape[0])
+        pad = torch.empty((2 * num, 2), dtype=torch.int64, device=ks.device)
+        out = torch.empty((num, 2), dtype=torch.int64, device=ks.device)
+        check(self.lib.cuot_cot_to_ot(self.ctx.handle, ks.data_ptr(), mr.data_ptr(), self.Delta.data_ptr(),
+                                      to_block(s), key_start, r.data_ptr(), m0.data_ptr(), m1.data_ptr(),
+                                      pad.data_ptr(), out.data_ptr(), num, self._stream()), "cuot_cot_to_ot")
+        return pad, out
+
+    def send_recv_rot(self, r, s, key_start: int = 0):
+        """COT::send_rot / recv_rot (cot.h:58-91): random OTs with the
+        receiver's choice r: (sender data0, data1, receiver data = data_r)."""
+        import torch
+
+        from . import to_block
+
+        ks, mr, _ = self.send_recv_cot(r)
+        num = int(r.shape[0])
+        d0, d1, out = (torch.empty((num, 2), dtype=torch.int64, device=ks.device) for _ in range(3))
+        check(self.lib.cuot_cot_to_rot(self.ctx.handle, ks.data_ptr(), mr.data_ptr(), self.Delta.data_ptr(),
+                                       to_block(s), key_start, d0.data_ptr(), d1.data_ptr(), out.data_ptr(), num,
+                                       self._stream()), "cuot_cot_to_rot")
+        return d0, d1, out
+
     # ---- malicious mode ----------------------------------------------------
     def set_malicious(self, on: bool = True):
         """MpcotReg::set_malicious: check every MPCOT (both parties)."""

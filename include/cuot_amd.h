@@ -433,6 +433,32 @@ int cuot_ferret_session_set_malicious(cuot_ferret_session *s, int on);
 int cuot_ferret_session_check(const cuot_ferret_session *s, int *ok, uint8_t *xprime, uint8_t *dig_a,
                               uint8_t *dig_b);
 
+/* ---- online derandomisation of COTs (both parties, asynchronous) -------------------
+ * data_s: the sender's COT blocks K, data_r: the receiver's M = K ^ c*Delta;
+ * one byte per bit (bool arrays, as the reference's interfaces).
+ * cuot_cot_online replaces FerretCOT::online_sender / online_recver
+ * (ferret_cot.hpp:277-295, the second half of send_cot / recv_cot): with the
+ * receiver's chosen bits b, bo = b ^ LSB(M) (the receiver's message) and
+ * K ^= bo*Delta in place, so that M = K ^ b*Delta.
+ * cuot_cot_to_ot replaces COT::send / COT::recv (cot.h:16-56) on such COTs:
+ * pad[2i] = H_i(K_i) ^ m0_i, pad[2i+1] = H_i(K_i ^ Delta) ^ m1_i (the sender's
+ * message), out_r[i] = pad[2i + r_i] ^ H_i(M_i) = m_{r_i, i}.
+ * cuot_cot_to_rot replaces COT::send_rot / recv_rot (cot.h:58-91):
+ * data0 = H_i(K_i), data1 = H_i(K_i ^ Delta), out_r = H_i(M_i).
+ * H_i = emp-tool's MITCCRH (mitccrh.h, emp-tool @44b1dde): AES under the key
+ * s ^ makeBlock(key_start + i, 0), XOR the input; s is the sender's random
+ * block (COT::send draws it), key_start the keys earlier calls consumed
+ * (0 for a fresh hash; emp-tool advances it by 8 per started group of 8). */
+int cuot_cot_online(cuot_ctx *ctx, cuot_block *data_s_d, const cuot_block *data_r_d, const uint8_t *b_d,
+                    const cuot_block *delta_d, uint8_t *bo_d, int64_t n, void *stream);
+int cuot_cot_to_ot(cuot_ctx *ctx, const cuot_block *data_s_d, const cuot_block *data_r_d,
+                   const cuot_block *delta_d, cuot_block s, uint64_t key_start, const uint8_t *r_d,
+                   const cuot_block *m0_d, const cuot_block *m1_d, cuot_block *pad_d, cuot_block *out_r_d,
+                   int64_t n, void *stream);
+int cuot_cot_to_rot(cuot_ctx *ctx, const cuot_block *data_s_d, const cuot_block *data_r_d,
+                    const cuot_block *delta_d, cuot_block s, uint64_t key_start, cuot_block *data0_d,
+                    cuot_block *data1_d, cuot_block *out_r_d, int64_t n, void *stream);
+
 /* ---- session state blob and pre-OT files (emp-ot byte layouts) --------------------
  * party: 1 = ALICE (sender, holds Delta), 2 = BOB (receiver).  The bootstrap of
  * M = k + h*t + 128 COTs plays emp-ot's ot_pre_data of n_pre blocks.

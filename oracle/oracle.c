@@ -1057,3 +1057,43 @@ int or_ferret_consist_final(const orb *V, const orb *W, const orb *chi_alpha, in
   or_sha256((const uint8_t *)&r1b, 16, dig_b);
   return memcmp(dig_a, dig_b, 32) == 0;
 }
+
+/* ==== online derandomisation (ferret_cot.hpp:277-295, cot.h:16-91) ==== */
+void or_cot_online(orb *data_s, const orb *data_r, const uint8_t *b, orb delta, uint8_t *bo, int64_t n) {
+  for (int64_t i = 0; i < n; i++) {
+    bo[i] = (uint8_t)((b[i] & 1) ^ (data_r[i].lo & 1));
+    if (bo[i]) data_s[i] = bx(data_s[i], delta);
+  }
+}
+
+/* MITCCRH (emp-tool mitccrh.h): H_i(x) = AES_{s ^ makeBlock(g0 + i, 0)}(x) ^ x */
+static orb mitccrh(orb s, uint64_t g, orb x) {
+  orb key = {s.lo, s.hi ^ g};
+  uint8_t rk[176];
+  or_aes_expand(&key, rk);
+  orb y;
+  or_aes_ecb(rk, &x, &y, 1, 0);
+  return bx(y, x);
+}
+
+void or_cot_to_ot(const orb *data_s, const orb *data_r, orb delta, orb s, uint64_t g0, const uint8_t *r,
+                  const orb *m0, const orb *m1, orb *pad, orb *out_r, int64_t n, int nthreads) {
+#pragma omp parallel for num_threads(nthreads)
+  for (int64_t i = 0; i < n; i++) {
+    const uint64_t g = g0 + (uint64_t)i;
+    pad[2 * i] = bx(mitccrh(s, g, data_s[i]), m0[i]);
+    pad[2 * i + 1] = bx(mitccrh(s, g, bx(data_s[i], delta)), m1[i]);
+    out_r[i] = bx(pad[2 * i + (r[i] & 1)], mitccrh(s, g, data_r[i]));
+  }
+}
+
+void or_cot_to_rot(const orb *data_s, const orb *data_r, orb delta, orb s, uint64_t g0, orb *data0, orb *data1,
+                   orb *out_r, int64_t n, int nthreads) {
+#pragma omp parallel for num_threads(nthreads)
+  for (int64_t i = 0; i < n; i++) {
+    const uint64_t g = g0 + (uint64_t)i;
+    data0[i] = mitccrh(s, g, data_s[i]);
+    data1[i] = mitccrh(s, g, bx(data_s[i], delta));
+    out_r[i] = mitccrh(s, g, data_r[i]);
+  }
+}

@@ -194,6 +194,17 @@ void or_ferret_consist_trees(const orb *leaves_s, const orb *leaves_r, int64_t n
  * LSB(pre_r); digests of XOR V ^ packing(pre_s ^ x' Delta) (ALICE) and
  * XOR W ^ packing(pre_r) (BOB), packing(d) = sum_j d_j x^j; returns 1 when
  * they agree */
+/* ---- online derandomisation -----------------------------------------------
+ * or_cot_online: FerretCOT::online_sender/online_recver (ferret_cot.hpp:277-295),
+ * bo = b ^ LSB(data_r), data_s ^= bo * Delta.  or_cot_to_ot / or_cot_to_rot:
+ * COT::send/recv and send_rot/recv_rot (cot.h:16-91) with emp-tool's MITCCRH
+ * H_i(x) = AES_{s ^ makeBlock(g0 + i, 0)}(x) ^ x (mitccrh.h, emp-tool
+ * @44b1dde restated: parity unpinned for the key derivation). */
+void or_cot_online(orb *data_s, const orb *data_r, const uint8_t *b, orb delta, uint8_t *bo, int64_t n);
+void or_cot_to_ot(const orb *data_s, const orb *data_r, orb delta, orb s, uint64_t g0, const uint8_t *r,
+                  const orb *m0, const orb *m1, orb *pad, orb *out_r, int64_t n, int nthreads);
+void or_cot_to_rot(const orb *data_s, const orb *data_r, orb delta, orb s, uint64_t g0, orb *data0, orb *data1,
+                   orb *out_r, int64_t n, int nthreads);
 int or_ferret_consist_final(const orb *V, const orb *W, const orb *chi_alpha, int64_t t, const orb *pre_s,
                             const orb *pre_r, orb delta, uint8_t xprime[128], uint8_t dig_a[32],
                             uint8_t dig_b[32]);

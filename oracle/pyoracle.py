@@ -398,3 +398,37 @@ def consist_final(V, W, chi_alpha, pre_s, pre_r, delta):
     ok = lib().or_ferret_consist_final(_p(V), _p(W), _p(ca), C.c_int64(V.shape[0]), _p(ps), _p(pr), as_orb(delta),
                                        xp, da, db)
     return bool(ok), np.frombuffer(bytes(xp), dtype=np.uint8).copy(), bytes(da), bytes(db)
+
+
+# ---- online derandomisation ------------------------------------------------------
+def cot_online(data_s, data_r, b, delta):
+    """(data_s with bo*Delta applied, bo)"""
+    ds = np.ascontiguousarray(data_s, dtype=np.uint64).reshape(-1, 2).copy()
+    dr = np.ascontiguousarray(data_r, dtype=np.uint64).reshape(-1, 2)
+    bb = np.ascontiguousarray(b, dtype=np.uint8)
+    bo = np.zeros(ds.shape[0], dtype=np.uint8)
+    lib().or_cot_online(_p(ds), _p(dr), _p(bb), as_orb(delta), _p(bo), C.c_int64(ds.shape[0]))
+    return ds, bo
+
+
+def cot_to_ot(data_s, data_r, delta, s, g0, r, m0, m1, nthreads=8):
+    ds = np.ascontiguousarray(data_s, dtype=np.uint64).reshape(-1, 2)
+    dr = np.ascontiguousarray(data_r, dtype=np.uint64).reshape(-1, 2)
+    n = ds.shape[0]
+    rr = np.ascontiguousarray(r, dtype=np.uint8)
+    a0 = np.ascontiguousarray(m0, dtype=np.uint64).reshape(-1, 2)
+    a1 = np.ascontiguousarray(m1, dtype=np.uint64).reshape(-1, 2)
+    pad, out = blocks(2 * n), blocks(n)
+    lib().or_cot_to_ot(_p(ds), _p(dr), as_orb(delta), as_orb(s), C.c_uint64(g0), _p(rr), _p(a0), _p(a1), _p(pad),
+                       _p(out), C.c_int64(n), C.c_int(nthreads))
+    return pad, out
+
+
+def cot_to_rot(data_s, data_r, delta, s, g0, nthreads=8):
+    ds = np.ascontiguousarray(data_s, dtype=np.uint64).reshape(-1, 2)
+    dr = np.ascontiguousarray(data_r, dtype=np.uint64).reshape(-1, 2)
+    n = ds.shape[0]
+    d0, d1, out = blocks(n), blocks(n), blocks(n)
+    lib().or_cot_to_rot(_p(ds), _p(dr), as_orb(delta), as_orb(s), C.c_uint64(g0), _p(d0), _p(d1), _p(out),
+                        C.c_int64(n), C.c_int(nthreads))
+    return d0, d1, out

@@ -200,11 +200,15 @@ def serve_leg(torch, cuot_amd, ctx, count, rate, budget_gb):
     total = sum(wi.n for _, wi in arr)
     budget = int(budget_gb * (1 << 30))
     res = {"config": {"workload": "serving", "requests": count, "rate_hz": rate,
-                      "sizes": "n = floor(2^(14 + 10u))", "kinds": "SOT / Ferret 50/50", "hbm_budget_bytes": budget}}
+                      "sizes": "n = floor(2^(14 + 10u))", "kinds": "SOT / Ferret 50/50", "hbm_budget_bytes": budget,
+                      "max_batch": 64, "max_batch_cots": 1 << 22, "streams": 2}}
+    torch.cuda.empty_cache()  # the earlier legs' cached blocks go back to the device
     for name, plain in (("served", False), ("fifo_plain", True)):
         torch.cuda.synchronize()
+        # batches are capped at 2^22 COTs so that a large request does not hold
+        # small ones behind it for a whole 2^24-COT encode
         lat, wall, nb = cuot_amd.serve_replay(ctx, insts, times, hbm_budget=budget, plain=plain,
-                                              max_batch=64, streams=2)
+                                              max_batch=64, streams=2, max_batch_cots=1 << 22)
         res[name] = {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
                      "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
                      "mean_ms": round(float(lat.mean()) * 1e3, 3),
@@ -363,6 +367,61 @@ def main():
                               "bootstrap_M": fc.M, "ot_limit": fc.ot_limit},
                    "ms_per_extend": round(sms, 4), "value": round(fc.ot_limit / (sms / 1e3), 1),
                    "unit": "usable COTs/s", "relation_ok": sok}
+        # malicious mode: the consistency check after every MPCOT
+        fc.set_malicious(True)
+        fc.extend_f2k()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            fc.extend_f2k()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        mms = e0.elapsed_time(e1) / args.steps
+        cok = fc.check_result()[0]
+        fc.set_malicious(False)
+        session["malicious"] = {"ms_per_extend": round(mms, 4), "value": round(fc.ot_limit / (mms / 1e3), 1),
+                                "unit": "usable COTs/s", "check_ok": cok,
+                                "check_ms": round(mms - sms, 4)}
+        # online derandomisation of one extend's usable COTs (both parties)
+        import ctypes as C
+
+        num = fc.ot_limit
+        g = torch.Generator().manual_seed(9)
+        bits = torch.randint(0, 2, (num,), dtype=torch.uint8, generator=g).to(stream.device)
+        m0 = torch.randint(-2**62, 2**62, (num, 2), dtype=torch.int64, generator=g).to(stream.device)
+        m1 = torch.randint(-2**62, 2**62, (num, 2), dtype=torch.int64, generator=g).to(stream.device)
+        bo = torch.empty(num, dtype=torch.uint8, device=stream.device)
+        pad = torch.empty((2 * num, 2), dtype=torch.int64, device=stream.device)
+        outr = torch.empty((num, 2), dtype=torch.int64, device=stream.device)
+        lib = ctx.lib
+        sblk = B((0x0123456789ABCDEF, 0x0FEDCBA987654321))
+        ks, mr = fc.out_s[:num], fc.out_r[:num]
+        dev_times = {}
+        for name, call in (
+                ("cot_online", lambda: lib.cuot_cot_online(ctx.handle, ks.data_ptr(), mr.data_ptr(), bits.data_ptr(),
+                                                           fc.Delta.data_ptr(), bo.data_ptr(), num,
+                                                           stream.cuda_stream)),
+                ("cot_to_ot", lambda: lib.cuot_cot_to_ot(ctx.handle, ks.data_ptr(), mr.data_ptr(), fc.Delta.data_ptr(),
+                                                         sblk, 0, bits.data_ptr(), m0.data_ptr(), m1.data_ptr(),
+                                                         pad.data_ptr(), outr.data_ptr(), num, stream.cuda_stream))):
+            cuot_amd.check(call(), name)
+            torch.cuda.synchronize()
+            e0.record(stream)
+            for _ in range(args.steps):
+                cuot_amd.check(call(), name)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            dev_times[name] = e0.elapsed_time(e1) / args.steps
+            if name == "cot_online" and (1 + args.steps) % 2 == 0:
+                cuot_amd.check(call(), name)  # each call toggles K by bo*Delta: end on the chosen COTs
+        oot = bool(torch.equal(outr, torch.where(bits.bool().unsqueeze(1), m1, m0)))
+        session["derandomisation"] = {
+            "n": num, "cot_online_ms": round(dev_times["cot_online"], 4),
+            "cot_to_ot_ms": round(dev_times["cot_to_ot"], 4),
+            "ots_per_s": round(num / ((dev_times["cot_online"] + dev_times["cot_to_ot"]) / 1e3), 1),
+            "form": "chosen-bit COT then chosen-message OT, both parties (3 AES + 1 key schedule per OT)",
+            "ot_ok": oot}
+        del bits, m0, m1, bo, pad, outr
         fc.close()
     # release the headline buffers before the other legs
     del ws, v, w, b

@@ -68,23 +68,16 @@ static inline int gf32_cantor_basis(uint32_t beta[32]) {
 // The accumulator is multiplied by t (a renaming of its 32 planes plus three
 // xors of the wrapped plane) and v is added for every set bit of c, high bit
 // first; v itself is never modified, so only v and the accumulator are live.
-// UNIFORM (c equal in every lane): two bits per step, adding v, v*t or
-// v + v*t (v*t differs from a renaming of v in three planes only) behind
-// scalar branches.  Otherwise one bit per step with per-lane masks.
+// One bit per step behind a branch on that bit of c: a scalar branch when c is
+// equal in every lane (UNIFORM), a lane-divergent one otherwise.  With
+// GF32_TWO_BIT, uniform constants take two bits per step, adding v, v*t or
+// v + v*t (v*t differs from a renaming of v in three planes only).
 #if defined(__HIP_DEVICE_COMPILE__)
 #define GF32_HD __device__ __forceinline__
-__device__ __forceinline__ uint32_t gf32_xa(uint32_t a, uint32_t b, uint32_t c) {
-  uint32_t r;  // a ^ (b & c) in one three-input operation (table 0x78)
-  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x78" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-  return r;
-}
-#else
-#ifdef __HIPCC__
+#elif defined(__HIPCC__)
 #define GF32_HD __host__ __device__ inline
 #else
 #define GF32_HD inline
-#endif
-GF32_HD uint32_t gf32_xa(uint32_t a, uint32_t b, uint32_t c) { return a ^ (b & c); }
 #endif
 
 // a ^ b ^ c; on the device one three-input bit operation (table 0x96) written
@@ -148,6 +141,8 @@ GF32_HD void gf32_top_steps_n(uint32_t* A, const uint32_t* v, uint32_t c) {
   }
 }
 
+// GF32_TWO_BIT = 1 selects two-bit digits for wave-uniform constants (fewer
+// instructions, but measured slower in the FFT pass: off by default)
 #ifndef GF32_TWO_BIT
 #define GF32_TWO_BIT 0
 #endif
@@ -170,54 +165,3 @@ GF32_HD void gf32_mulc_top(const uint32_t* v, uint32_t c, uint32_t* acc) {
     gf32_top_steps_n<32>(acc, v, c);
   }
 }
-
-#ifdef __HIPCC__
-// ---- device -------------------------------------------------------------------
-// y ^= c * x for 32 bitsliced rows (Horner form: w runs through x * t^i; a
-// multiplication by t is a rotation of the 32 planes, which full unrolling
-// turns into register renaming, plus the reduction t^32 = t^7 + t^3 + t^2 + 1
-// of the plane that wraps).  UNIFORM: c is the same in every lane of the wave
-// (branches on its bits are scalar); otherwise per-lane bit masks.
-// a ^ (b & c) as one gfx950 three-input bit operation (truth table 0x78 with
-// index src0*4 + src1*2 + src2); written as asm so that the compiler keeps
-// one instruction per term instead of re-associating the xor chain.
-__device__ __forceinline__ uint32_t gf32_xor_and(uint32_t a, uint32_t b, uint32_t c) {
-  uint32_t r;
-  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x78" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-  return r;
-}
-
-template <bool UNIFORM>
-__device__ __forceinline__ void gf32_mulc_acc(const uint32_t* __restrict__ x, uint32_t c, uint32_t* __restrict__ y) {
-  uint32_t w[32];
-#pragma unroll
-  for (int j = 0; j < 32; j++) w[j] = x[j];
-#pragma unroll
-  for (int i = 0; i < 32; i++) {
-    if (UNIFORM) {
-      if ((c >> i) & 1u) {
-#pragma unroll
-        for (int j = 0; j < 32; j++) y[j] ^= w[(j - i) & 31];
-      }
-    } else {
-      const uint32_t m = 0u - ((c >> i) & 1u);
-#pragma unroll
-      for (int j = 0; j < 32; j++) y[j] = gf32_xor_and(y[j], w[(j - i) & 31], m);
-    }
-    // w <- w * t: physical slot (31 - i) & 31 holds the plane that wraps to
-    // t^0; fold it into t^2, t^3, t^7 (slots relative to the rotation).
-    const uint32_t top = w[(31 - i) & 31];
-    w[(2 - i - 1) & 31] ^= top;
-    w[(3 - i - 1) & 31] ^= top;
-    w[(7 - i - 1) & 31] ^= top;
-  }
-}
-
-// y = c * x
-template <bool UNIFORM>
-__device__ __forceinline__ void gf32_mulc(const uint32_t* __restrict__ x, uint32_t c, uint32_t* __restrict__ y) {
-#pragma unroll
-  for (int j = 0; j < 32; j++) y[j] = 0;
-  gf32_mulc_acc<UNIFORM>(x, c, y);
-}
-#endif

@@ -92,6 +92,7 @@ struct cuot_ferret_session {
   int64_t npre = 0, M = 0, ot_limit = 0, round = 0;
   bool seeded = false;  // the MPCOT half of the current extend has run
   bool malicious = false;  // run the consistency check after each MPCOT
+  bool checked = false;    // a consistency check has been enqueued
   blk* vwc = nullptr;      // per-tree V, W, chi_alpha (3 t blocks)
   uint8_t* chk = nullptr;  // x' (128), digests (2 x 32), ok (int32)
   char* mem = nullptr;
@@ -132,6 +133,7 @@ size_t carve_session(cuot_ferret_session* s, char* base) {
 // bootstrap COTs (mpcot_reg.h:94-95: pre_cot_data = ot_input)
 hipError_t consistency(cuot_ferret_session* s, const blk* out_s, const blk* out_r, hipStream_t st) {
   const cuot_ferret_config& cfg = s->plan.cfg;
+  s->checked = true;
   return cuot_launch_consist(out_s, out_r, cfg.n, cfg.t, cfg.h, s->ssum, s->bits, s->boot_s, s->boot_r, s->delta,
                              s->vwc, s->chk, st);
 }
@@ -460,6 +462,10 @@ extern "C" int cuot_ferret_session_set_malicious(cuot_ferret_session* s, int on)
 extern "C" int cuot_ferret_session_check(const cuot_ferret_session* s, int* ok, uint8_t* xprime, uint8_t* dig_a,
                                          uint8_t* dig_b) {
   CUOT_CHECK_ARG(s && ok, "NULL argument");
+  if (!s->checked) {
+    cuot_set_last_error("no consistency check has run on this session", __FILE__, __LINE__);
+    return CUOT_ERR_STATE;
+  }
   uint8_t h[196];
   CUOT_HIP_TRY(hipSetDevice(s->ctx->device));
   CUOT_HIP_TRY(hipDeviceSynchronize());

@@ -429,7 +429,8 @@ int cuot_ferret_session_free(cuot_ferret_session *s);
  * punctured leaf's chi when that leaf is not stored. */
 int cuot_ferret_session_set_malicious(cuot_ferret_session *s, int on);
 /* the last check (waits for the device): ok = 1 when ALICE's and BOB's
- * digests agree; x' (128 bytes, BOB's message), digests (32 bytes each) */
+ * digests agree; x' (128 bytes, BOB's message), digests (32 bytes each);
+ * CUOT_ERR_STATE before any check has run */
 int cuot_ferret_session_check(const cuot_ferret_session *s, int *ok, uint8_t *xprime, uint8_t *dig_a,
                               uint8_t *dig_b);
 

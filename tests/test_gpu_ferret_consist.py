@@ -40,7 +40,7 @@ def test_check_matches_oracle(ctx, n, k, t, h):
 def test_corrupted_leaf_fails(ctx):
     import torch
 
-    from cuot_amd import _lib
+    from cuot_amd import CuotError, _lib
     from cuot_amd.ferret import FerretCOT
 
     print("ferret consistency, corrupted leaf", flush=True)
@@ -50,6 +50,8 @@ def test_corrupted_leaf_fails(ctx):
     st = torch.cuda.current_stream().cuda_stream
     args = (fc.handle, fc.out_s.data_ptr(), fc.out_r.data_ptr())
     assert lib.cuot_ferret_session_extend_stages(*args, 4, st) != 0  # check before any MPCOT
+    with pytest.raises(CuotError):  # no result before a check ran
+        fc.check_result()
     assert lib.cuot_ferret_session_extend_stages(*args, 1, st) == 0
     assert lib.cuot_ferret_session_extend_stages(*args, 4, st) == 0
     ok, _, da, db = fc.check_result()

@@ -451,7 +451,12 @@ __global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(2))
       }
     }
   };
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  // workgroups are dealt to the 8 XCDs in turn: renumber them so that one
+  // XCD's workgroups take neighbouring tiles, whose column blocks share
+  // 128-byte lines in that XCD's L2
+  const int64_t G = gridDim.x, bx = blockIdx.x;
+  const int64_t slot = (G & 7) ? bx : (bx & 7) * (G >> 3) + (bx >> 3);
+  for (int64_t tile = slot; tile < ntiles; tile += G) {
     load_tile(tile);
     const int64_t cb = tile & (cblocks - 1), hib = tile >> (a.lo - a.lgC);
 #pragma unroll

@@ -436,11 +436,20 @@ __global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(2))
   const int n4 = TC * 32;
   const uint4* X4 = (const uint4*)a.X;
   uint4 v[FFT_LD];
+  // staging addresses come from an opaque copy of the lane index, so that the
+  // compiler recomputes them per phase instead of holding them through the
+  // layers
+  auto opaque_tid = [] {
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+  };
   auto load_tile = [&](int64_t tl) {
     const int64_t lcb = tl & (cblocks - 1), lhib = tl >> (a.lo - a.lgC);
+    const int tid = opaque_tid();
 #pragma unroll
     for (int u = 0; u < FFT_LD; u++) {
-      const int f = (int)threadIdx.x + u * (int)blockDim.x;
+      const int f = tid + u * FFT_THREADS;
       v[u] = z4();
       if (f < n4) {
         const int c = f & (a.C - 1);
@@ -456,8 +465,11 @@ __global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(2))
   // 128-byte lines in that XCD's L2
   const int64_t G = gridDim.x, bx = blockIdx.x;
   const int64_t slot = (G & 7) ? bx : (bx & 7) * (G >> 3) + (bx >> 3);
+  // A tile's loads are issued in the previous tile's write-back phase (after
+  // its results have left the LDS for registers), so that the HBM reads and
+  // writes of consecutive tiles are in flight together.
+  if (slot < ntiles) load_tile(slot);
   for (int64_t tile = slot; tile < ntiles; tile += G) {
-    load_tile(tile);
     const int64_t cb = tile & (cblocks - 1), hib = tile >> (a.lo - a.lgC);
 #pragma unroll
     for (int u = 0; u < FFT_LD; u++) {
@@ -488,16 +500,30 @@ __global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(2))
           fft_layer(a.lo, a.R, a.C, a.lgC, lds, hib, cb, k, inv);
       }
     }
+    uint4 w[FFT_LD];
+    const int tid = opaque_tid();
+#pragma unroll
+    for (int u = 0; u < FFT_LD; u++) {
+      const int f = tid + u * FFT_THREADS;
+      w[u] = z4();
+      if (f < n4) {
+        const int c = f & (a.C - 1);
+        const int pt = f >> a.lgC;
+        const int t = pt & (T - 1);
+        w[u] = *(const uint4*)&lds[(t * a.C + c) * FFT_STRIDE + (pt >> a.R) * 4];
+      }
+    }
+    if (tile + G < ntiles) load_tile(tile + G);
     uint4* Y4 = (uint4*)a.X;
 #pragma unroll
     for (int u = 0; u < FFT_LD; u++) {
-      const int f = (int)threadIdx.x + u * (int)blockDim.x;
+      const int f = tid + u * FFT_THREADS;
       if (f < n4) {
         const int c = f & (a.C - 1);
         const int pt = f >> a.lgC;
         const int t = pt & (T - 1), p = pt >> a.R;
         const int64_t idx = (hib << (a.lo + a.R)) | ((int64_t)t << a.lo) | (cb * a.C + c);
-        Y4[(int64_t)p * a.M + idx] = *(const uint4*)&lds[(t * a.C + c) * FFT_STRIDE + p * 4];
+        Y4[(int64_t)p * a.M + idx] = w[u];
       }
     }
     __syncthreads();

@@ -110,19 +110,6 @@ __global__ void qc_pack_x(const QcIo* __restrict__ io, QcIo io1, uint4* __restri
   }
 }
 
-// single-row variant for a: row 0 (bit 0 of word 0) = bit (16c + b) of a
-__global__ void qc_pack_a(const uint8_t* __restrict__ a, int64_t n, uint4* __restrict__ X, int64_t M) {
-  const int64_t half = M >> 1;
-  const int64_t total = half * 16;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t c = t >> 4;
-    const int b = (int)(t & 15);
-    const int64_t src = 16 * c + b;
-    uint32_t bit = src < n ? (a[src >> 3] >> (src & 7)) & 1u : 0u;
-    X[(int64_t)b * M + c] = make_uint4(bit, 0, 0, 0);
-  }
-}
-
 // ---- basis conversion -------------------------------------------------------
 // Division by s_k on chunks of 2^(k+1) (forward, monomial -> novel basis):
 //   q_i = hi_i ^ XOR_{e in T_k, e > i} hi_{i + half - e}
@@ -245,11 +232,11 @@ __global__ __launch_bounds__(1024) void qc_conv_rows(uint32_t* __restrict__ Bw, 
                                                      int64_t tiles_per_inst, int64_t inst_words, int mp,
                                                      int inverse, int fold) {
   extern __shared__ uint32_t sw[];
+  const int n4 = (int)(tile_words >> 2);
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t* g = Bw + (tile / tiles_per_inst) * inst_words + (tile % tiles_per_inst) * tile_words;
     if ((tile_words & 3) == 0) {
       // 16-byte accesses, CR_LD in flight per lane
-      const int n4 = (int)(tile_words >> 2);
       const uint4* g4 = (const uint4*)g;
       for (int f0 = threadIdx.x; f0 < n4; f0 += blockDim.x * CR_LD) {
         uint4 v[CR_LD];
@@ -302,7 +289,6 @@ __global__ __launch_bounds__(1024) void qc_conv_rows(uint32_t* __restrict__ Bw, 
         g4[f] = acc;
       }
     } else if ((tile_words & 3) == 0) {
-      const int n4 = (int)(tile_words >> 2);
       uint4* g4 = (uint4*)g;
       for (int f = threadIdx.x; f < n4; f += blockDim.x) g4[f] = ((const uint4*)sw)[f];
     } else {
@@ -605,15 +591,6 @@ __global__ void qcs_fft_layer(const QcPrep* __restrict__ pr, QcPrep pr1, int64_t
     const uint32_t xu = A[u] ^ gf32_mul_dev(c, A[v]);
     A[u] = xu;
     A[v] = A[v] ^ xu;
-  }
-}
-
-// A^[e] = element of row 0: bit j = bit 0 of plane j word 0
-__global__ void qc_extract_row0(const uint32_t* __restrict__ X, int64_t M, uint32_t* __restrict__ ahat) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < M; e += (int64_t)gridDim.x * blockDim.x) {
-    uint32_t v = 0;
-    for (int p = 0; p < 32; p++) v |= (X[((int64_t)p * M + e) * 4] & 1u) << p;
-    ahat[e] = v;
   }
 }
 

@@ -48,12 +48,28 @@ CUOT_DEV uint32_t tlook(const AesLds& a, uint32_t b) {
 
 CUOT_DEV uint32_t rotl32(uint32_t x, int s) { return (x << s) | (x >> (32 - s)); }
 
+// a ^ b ^ c as one three-input bit operation (table 0x96) on VGPR operands;
+// the round key (uniform in some kernels, per lane in others) is added by a
+// plain XOR, which takes it from either register file
+#ifndef AES_XOR3
+#define AES_XOR3 1
+#endif
+CUOT_DEV uint32_t aes_x3(uint32_t a, uint32_t b, uint32_t c) {
+#if AES_XOR3
+  uint32_t r;
+  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+#else
+  return a ^ b ^ c;
+#endif
+}
+
 CUOT_DEV uint32_t aes_col(const AesLds& a, uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t k) {
   uint32_t t0 = tlook(a, w0 & 0xffu);
   uint32_t t1 = tlook(a, (w1 >> 8) & 0xffu);
   uint32_t t2 = tlook(a, (w2 >> 16) & 0xffu);
   uint32_t t3 = tlook(a, w3 >> 24);
-  return t0 ^ rotl32(t1, 8) ^ rotl32(t2, 16) ^ rotl32(t3, 24) ^ k;
+  return aes_x3(t0, rotl32(t1, 8), rotl32(t2, 16)) ^ rotl32(t3, 24) ^ k;
 }
 
 // Last round: S-box bytes recovered from T0 (T0[x] bytes are 2s, s, s, 3s).

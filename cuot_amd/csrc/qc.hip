@@ -552,9 +552,10 @@ __global__ void qcs_hi_fix(const QcPrep* __restrict__ pr, QcPrep pr1, int64_t Mp
   const int ne = lower_terms(k, e);
   if (ne == 0) return;
   const int64_t emax = e[ne - 1], half = (int64_t)1 << k;
-  const int64_t total = (Mp >> (k + 1)) * emax;
+  const int le = __builtin_ctz((uint32_t)emax);
+  const int64_t total = (Mp >> (k + 1)) << le;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t g = t / emax, i = t % emax;
+    const int64_t g = t >> le, i = t & (emax - 1);
     uint32_t* h = A + g * 2 * half + half;
     uint32_t v = h[i];
     for (int r = 0; r < ne; r++)
@@ -592,6 +593,72 @@ __global__ void qcs_fft_layer(const QcPrep* __restrict__ pr, QcPrep pr1, int64_t
     A[u] = xu;
     A[v] = A[v] ^ xu;
   }
+}
+
+// The whole preparation of one polynomial in one workgroup's LDS when its
+// 2^m words fit (small codes): the same pack, conversion levels and FFT layers
+// as the kernels above, separated by barriers instead of launches.
+constexpr int PREP_LDS_MMAX = 15;  // 2^15 words = 128 KB
+
+__global__ __launch_bounds__(1024) void qcs_prepare_lds(const QcPrep* __restrict__ pr, QcPrep pr1, int m) {
+  extern __shared__ uint32_t L[];
+  const QcPrep q = pr ? pr[blockIdx.x] : pr1;
+  const int64_t M = (int64_t)1 << m, Mh = M >> 1;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  for (int64_t c = tid; c < M; c += nth) {
+    uint32_t v = 0;
+    if (c < Mh) {
+      for (int b = 0; b < 16; b++) {
+        const int64_t src = 16 * c + b;
+        if (src < q.n) v |= (uint32_t)((q.a_bits[src >> 3] >> (src & 7)) & 1u) << b;
+      }
+    }
+    L[c] = v;
+  }
+  __syncthreads();
+  // monomial -> novel basis on the lower half (division by s_k, k = m-2 .. 0)
+  for (int k = m - 2; k >= 0; k--) {
+    uint32_t e[16];
+    const int ne = lower_terms(k, e);
+    if (ne == 0) continue;
+    const int64_t half = (int64_t)1 << k;
+    const int64_t emax = e[ne - 1];
+    const int le = __builtin_ctz((uint32_t)emax);
+    for (int64_t t = tid; t < ((Mh >> (k + 1)) << le); t += nth) {
+      const int64_t g = t >> le, i = t & (emax - 1);
+      uint32_t* h = L + g * 2 * half + half;
+      uint32_t v = h[i];
+      for (int r = 0; r < ne; r++)
+        if ((int64_t)e[r] > i) v ^= h[i + half - e[r]];
+      h[i] = v;
+    }
+    __syncthreads();
+    for (int64_t t = tid; t < (Mh >> 1); t += nth) {
+      const int64_t g = t >> k, j = t & (half - 1);
+      uint32_t* l = L + g * 2 * half;
+      uint32_t v = l[j];
+      for (int r = 0; r < ne; r++) {
+        if ((int64_t)e[r] > j) break;
+        v ^= l[half + j - e[r]];
+      }
+      l[j] = v;
+    }
+    __syncthreads();
+  }
+  // FFT layers m-1 .. 0 (the top one on a zero upper half copies the lower)
+  for (int k = m - 1; k >= 0; k--) {
+    const int64_t half = (int64_t)1 << k;
+    for (int64_t b = tid; b < Mh; b += nth) {
+      const int64_t g = b >> k, j = b & (half - 1);
+      const int64_t u = g * 2 * half + j, v = u + half;
+      const uint32_t c = chunk_const(c_beta, (uint64_t)g);
+      const uint32_t xu = L[u] ^ gf32_mul_dev(c, L[v]);
+      L[u] = xu;
+      L[v] = L[v] ^ xu;
+    }
+    __syncthreads();
+  }
+  for (int64_t c = tid; c < M; c += nth) q.ahat[c] = L[c];
 }
 
 // ---- output: top inverse level + overlap-add + fold + x0 ---------------------
@@ -755,6 +822,9 @@ hipError_t configure_lds() {
   }
   hipError_t e = hipFuncSetAttribute((const void*)qc_fft_pass, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (FFT_TC * FFT_STRIDE + CTAB_WORDS) * 4);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)qcs_prepare_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          ((int64_t)1 << PREP_LDS_MMAX) * 4);
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute((const void*)qc_conv_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
                           ((int64_t)1 << ROW_MP_MAX) / 8);
@@ -996,6 +1066,12 @@ hipError_t prepare_batch(const QcPrep* pr, QcPrep pr1, int B, int m, hipStream_t
   if (e != hipSuccess) return e;
   const int64_t M = (int64_t)1 << m;
   const int64_t Mh = M >> 1;
+  // several polynomials: one workgroup each; a single one keeps the
+  // grid-wide launches (one workgroup would be its critical path)
+  if (m <= PREP_LDS_MMAX && B > 1) {
+    hipLaunchKernelGGL(qcs_prepare_lds, dim3((unsigned)B), dim3(1024), (size_t)M * 4, st, pr, pr1, m);
+    return hipGetLastError();
+  }
   const dim3 blk(256);
   hipLaunchKernelGGL(qcs_pack, dim3(grid_for(M), (unsigned)B), blk, 0, st, pr, pr1, M);
   for (int k = m - 2; k >= 0; k--) {

@@ -20,7 +20,7 @@ MODE_SOT = 0
 MODE_FERRET = 1
 
 __all__ = ["Context", "Batch", "Server", "ServerConfig", "serve_replay", "Block", "GgmParams", "LpnParams", "SotConfig", "FerretConfig", "Instance",
-           "CuotError", "MODE_SOT", "MODE_FERRET", "BATCH_GRAPH", "to_block", "load"]
+           "CuotError", "MODE_SOT", "MODE_FERRET", "BATCH_GRAPH", "BATCH_OVERLAP", "to_block", "load"]
 
 
 def to_block(b) -> Block:
@@ -153,6 +153,7 @@ BATCH_STAGES = [("setup", 1), ("qc_prepare", 2), ("ggm_sender", 4), ("ggm_recver
                 ("lpn", 64)]
 BATCH_ALL = 127
 BATCH_GRAPH = 1 << 16
+BATCH_OVERLAP = 1 << 17
 
 
 class Batch:
@@ -192,8 +193,8 @@ class Batch:
         check(self.lib.cuot_batch_bind(h, self.ws.data_ptr(), self.ws_bytes, o0, o1, bb, self.delta.data_ptr()),
               "cuot_batch_bind")
 
-    def run(self, stages: int = BATCH_ALL, graph: bool = False, stream=None):
-        flags = stages | (BATCH_GRAPH if graph else 0)
+    def run(self, stages: int = BATCH_ALL, graph: bool = False, stream=None, overlap: bool = False):
+        flags = stages | (BATCH_GRAPH if graph else 0) | (BATCH_OVERLAP if overlap else 0)
         check(self.lib.cuot_batch_run(self.handle, flags, _stream_ptr(stream)), "cuot_batch_run")
 
     def close(self):

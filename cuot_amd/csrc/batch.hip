@@ -46,6 +46,7 @@ struct QcChunk {
   int m = 0;
   std::vector<std::pair<int, int>> parties;  // (instance, 0 sender / 1 receiver)
   size_t io_off = 0, ahat_off = 0;
+  bool aux = false;  // second stream under CUOT_BATCH_OVERLAP (own scratch region)
 };
 
 }  // namespace
@@ -63,6 +64,7 @@ struct cuot_batch {
   size_t lpn_tab_off = 0, lpn_start_off = 0;
   int64_t lpn_wgs = 0;
   size_t tables_bytes = 0, qc_scratch_off = 0, qc_scratch_bytes = 0, ws_bytes = 0;
+  size_t qc_scratch2_off = 0;  // scratch of the chunks marked aux
   size_t qc_chunk_bytes = kDefaultQcChunk;
   // bound state
   char* ws = nullptr;
@@ -206,11 +208,27 @@ int plan_batch(cuot_batch* b) {
     b->inst_off[i] = off;
     off = align256(off + (b->is_sot(i) ? b->sot[i].carve(nullptr) : b->fer[i].carve(nullptr)));
   }
-  b->qc_scratch_bytes = 0;
-  for (const QcChunk& c : b->qc)
-    b->qc_scratch_bytes = std::max(b->qc_scratch_bytes, cuot_qc_batch_scratch_bytes(c.m, (int)c.parties.size()));
+  // Chunks by decreasing work (parties x M log M) go to the two streams in
+  // turn, the largest to the main stream: with CUOT_BATCH_OVERLAP the aux
+  // chunks run on the second stream with their own scratch region.
+  std::vector<int> order(b->qc.size());
+  for (size_t j = 0; j < order.size(); j++) order[j] = (int)j;
+  auto work = [&](int j) { return (double)b->qc[j].parties.size() * (double)((int64_t)1 << b->qc[j].m) * b->qc[j].m; };
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+  size_t scr1 = 0, scr2 = 0;
+  for (size_t r = 0; r < order.size(); r++) {
+    QcChunk& c = b->qc[order[r]];
+    c.aux = (r & 1) != 0;
+    const size_t need = cuot_qc_batch_scratch_bytes(c.m, (int)c.parties.size());
+    if (c.aux)
+      scr2 = std::max(scr2, need);
+    else
+      scr1 = std::max(scr1, need);
+  }
+  b->qc_scratch_bytes = scr1;
   b->qc_scratch_off = off;
-  b->ws_bytes = off + b->qc_scratch_bytes + 256;
+  b->qc_scratch2_off = align256(off + scr1);
+  b->ws_bytes = b->qc_scratch2_off + scr2 + 256;
   return CUOT_OK;
 }
 
@@ -305,10 +323,17 @@ int build_tables(cuot_batch* b, std::vector<char>& img) {
   return CUOT_OK;
 }
 
-hipError_t launch_all(cuot_batch* b, uint32_t stages, hipStream_t st) {
+// ax (non-null, all stages): the context's auxiliary stream takes the A^
+// preparation, the sparse encodes, the Ferret LPN and the QC chunks marked
+// aux, forked after the setup and joined at the end (graph capture follows
+// the events), so that the LPN gathers and small encodes run beside the large
+// encodes.
+hipError_t launch_all(cuot_batch* b, uint32_t stages, hipStream_t st, hipStream_t ax) {
   hipError_t e;
   const int count = (int)b->inst.size();
   char* ws = b->ws;
+  const bool two = ax != nullptr && stages == CUOT_BATCH_STAGE_ALL;
+  hipEvent_t* ev = b->ctx->ev;
   if (stages & CUOT_BATCH_STAGE_SETUP) {
     for (int i = 0; i < count; i++) {
       if ((e = b->is_sot(i) ? b->sot[i].run_setup(st) : b->fer[i].run_setup(st))) return e;
@@ -318,10 +343,25 @@ hipError_t launch_all(cuot_batch* b, uint32_t stages, hipStream_t st) {
       }
     }
   }
+  hipStream_t sp = st;  // preparation, sparse encodes, LPN
+  if (two) {
+    if ((e = hipEventRecord(ev[0], st))) return e;
+    if ((e = hipStreamWaitEvent(ax, ev[0], 0))) return e;
+    sp = ax;
+  }
   if (stages & CUOT_BATCH_STAGE_QC_PREPARE)
     for (const PrepGroup& pg : b->prep)
-      if ((e = cuot_launch_qc_prepare_batch((const QcPrep*)(ws + pg.off), (int)pg.members.size(), pg.m, st)))
+      if ((e = cuot_launch_qc_prepare_batch((const QcPrep*)(ws + pg.off), (int)pg.members.size(), pg.m, sp)))
         return e;
+  if (two) {
+    if ((e = hipEventRecord(ev[1], sp))) return e;
+  }
+  if (stages & CUOT_BATCH_STAGE_SPARSE)
+    for (int i = 0; i < count; i++) {
+      if (!b->is_sot(i)) continue;
+      const SotPlan& p = b->sot[i];
+      if ((e = cuot_launch_sparse(p.a_words, p.cfg.n, p.pos, (int)p.cfg.t, b->bits[i], sp))) return e;
+    }
   for (int r = 0; r < 2; r++) {
     if (!(stages & (r ? CUOT_BATCH_STAGE_GGM_RECVER : CUOT_BATCH_STAGE_GGM_SENDER))) continue;
     for (const GgmGroup& g : b->ggm) {
@@ -340,25 +380,32 @@ hipError_t launch_all(cuot_batch* b, uint32_t stages, hipStream_t st) {
       }
     }
   }
-  if (stages & CUOT_BATCH_STAGE_QC)
-    for (const QcChunk& c : b->qc)
-      if ((e = cuot_launch_qc_encode_batch((const QcIo*)(ws + c.io_off), (const uint32_t* const*)(ws + c.ahat_off),
-                                           (int)c.parties.size(), c.m, ws + b->qc_scratch_off, st)))
-        return e;
-  if (stages & CUOT_BATCH_STAGE_SPARSE)
-    for (int i = 0; i < count; i++) {
-      if (!b->is_sot(i)) continue;
-      const SotPlan& p = b->sot[i];
-      if ((e = cuot_launch_sparse(p.a_words, p.cfg.n, p.pos, (int)p.cfg.t, b->bits[i], st))) return e;
-    }
+  if (two) {
+    if ((e = hipEventRecord(ev[2], st))) return e;
+    if ((e = hipStreamWaitEvent(ax, ev[2], 0))) return e;
+    if ((e = hipStreamWaitEvent(st, ev[1], 0))) return e;
+  }
   if ((stages & CUOT_BATCH_STAGE_LPN) && !b->ferret_ids.empty()) {
     for (int i : b->ferret_ids)
-      if ((e = b->fer[i].fill_kkp(st))) return e;
-  }
-  if ((stages & CUOT_BATCH_STAGE_LPN) && !b->ferret_ids.empty())
+      if ((e = b->fer[i].fill_kkp(sp))) return e;
     if ((e = cuot_launch_lpn_batch(ws + b->lpn_tab_off, (const int64_t*)(ws + b->lpn_start_off),
-                                   (int)b->ferret_ids.size(), b->lpn_wgs, 2, st)))
+                                   (int)b->ferret_ids.size(), b->lpn_wgs, 2, sp)))
       return e;
+  }
+  // QC chunks: on one stream all share a scratch region at a time; under the
+  // overlap the aux chunks use the second region on the second stream
+  if (stages & CUOT_BATCH_STAGE_QC)
+    for (const QcChunk& c : b->qc) {
+      const bool on_ax = two && c.aux;
+      char* scr = ws + (c.aux ? b->qc_scratch2_off : b->qc_scratch_off);
+      if ((e = cuot_launch_qc_encode_batch((const QcIo*)(ws + c.io_off), (const uint32_t* const*)(ws + c.ahat_off),
+                                           (int)c.parties.size(), c.m, scr, on_ax ? ax : st)))
+        return e;
+    }
+  if (two) {
+    if ((e = hipEventRecord(ev[3], ax))) return e;
+    if ((e = hipStreamWaitEvent(st, ev[3], 0))) return e;
+  }
   return hipSuccess;
 }
 
@@ -445,15 +492,21 @@ extern "C" int cuot_batch_run(cuot_batch* b, uint32_t stages, void* stream) {
   CUOT_HIP_TRY(hipSetDevice(b->ctx->device));
   hipStream_t st = b->ctx->st(stream);
   const uint32_t work = stages & CUOT_BATCH_STAGE_ALL;
+  hipStream_t ax = nullptr;
+  if (stages & CUOT_BATCH_OVERLAP) {
+    CUOT_HIP_TRY(b->ctx->ensure_aux());
+    ax = b->ctx->aux;
+  }
   if (!(stages & CUOT_BATCH_GRAPH)) {
-    CUOT_HIP_TRY(launch_all(b, work, st));
+    CUOT_HIP_TRY(launch_all(b, work, st, ax));
     return CUOT_OK;
   }
   CUOT_CHECK_ARG(st != nullptr, "graph replay needs a stream (not the null stream)");
-  if (!b->exec || b->graph_stages != work) {
+  const uint32_t key = work | (stages & CUOT_BATCH_OVERLAP);
+  if (!b->exec || b->graph_stages != key) {
     b->drop_graph();
     CUOT_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    hipError_t e = launch_all(b, work, st);
+    hipError_t e = launch_all(b, work, st, ax);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(st, &g);
     if (e == hipSuccess) e = e2;
@@ -467,7 +520,7 @@ extern "C" int cuot_batch_run(cuot_batch* b, uint32_t stages, void* stream) {
       b->drop_graph();
       CUOT_HIP_TRY(e3);
     }
-    b->graph_stages = work;
+    b->graph_stages = key;
   }
   CUOT_HIP_TRY(hipGraphLaunch(b->exec, st));
   return CUOT_OK;

@@ -296,7 +296,10 @@ enum {
   CUOT_BATCH_STAGE_SPARSE = 32,    /* SOT choice bits */
   CUOT_BATCH_STAGE_LPN = 64,       /* Ferret LPN of both parties */
   CUOT_BATCH_STAGE_ALL = 127,
-  CUOT_BATCH_GRAPH = 1 << 16       /* run flag: replay a captured HIP graph */
+  CUOT_BATCH_GRAPH = 1 << 16,      /* run flag: replay a captured HIP graph */
+  CUOT_BATCH_OVERLAP = 1 << 17     /* run flag (all stages): A^ preparation, sparse encodes and Ferret LPN
+                                      on the context's auxiliary stream, forked and joined around the run;
+                                      calls on one context come from one host thread at a time */
 };
 
 typedef struct cuot_batch cuot_batch;

@@ -110,3 +110,31 @@ def test_batch_graph_replays(ctx):
         for i, (wi, _) in enumerate(mix):
             assert _same(_batch_out(bt, i, wi), ref[i]), f"replay {rep} instance {i} differs"
     bt.close()
+
+
+def test_batch_overlap_matches(ctx):
+    """CUOT_BATCH_OVERLAP (preparation, sparse encodes and LPN on the second
+    stream): eager and graph replays byte-identical to the one-stream run."""
+    import torch
+
+    import cuot_amd
+
+    mix = _mix()
+    print("batch with the two-stream overlap", flush=True)
+    bt = cuot_amd.Batch(ctx, [ins for _, ins in mix])
+    bt.run()
+    torch.cuda.synchronize()
+    ref = [_batch_out(bt, i, wi) for i, (wi, _) in enumerate(mix)]
+    s = torch.cuda.Stream()
+    for rep, graph in enumerate((False, True, True, True)):
+        for t in bt.out0 + bt.out1:
+            t.fill_(-1)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            bt.run(graph=graph, stream=s, overlap=True)
+        s.synchronize()
+        junk = torch.randn(1 << 20, device="cuda").sum().item()
+        assert junk == junk
+        for i, (wi, _) in enumerate(mix):
+            assert _same(_batch_out(bt, i, wi), ref[i]), f"run {rep} (graph={graph}) instance {i} differs"
+    bt.close()

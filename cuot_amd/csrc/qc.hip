@@ -45,7 +45,7 @@ constexpr int FFT_LD = 16;  // 16-byte loads in flight per lane (tile staging)
 
 constexpr int FFT_WGS = 512;  // FFT pass grid: two resident WGs on each of the 256 CUs
 constexpr int CR_LD = 8;    // same for the bit-row conversion tiles
-// the row-layout conversion handles strings of 2^5 .. 2^20 coefficients (128 KB LDS)
+// the row-layout conversion handles strings of 2^5 .. 2^20 coefficients (128 KB, padded 144 KB, of LDS)
 constexpr int ROW_MP_MIN = 5, ROW_MP_MAX = 20;
 
 struct Beta {
@@ -228,9 +228,12 @@ struct BarrierSync {
 // of s_mp, rotations cyclic over the 2^mp-bit string.  The lower half r then
 // satisfies r ^ T(q) = (F mod X^(2^mp) + 1) for F = r + s_mp q, the product
 // folded at half its length (s_mp = X^(2^mp) + sum_E X^e).
+// PAD (strings of >= 2^10 coefficients): the string sits in LDS with four pad
+// words after every 32, and levels 9..0 run per 1024-bit block in registers.
+template <bool PAD, bool INV>
 __global__ __launch_bounds__(1024) void qc_conv_rows(uint32_t* __restrict__ Bw, int64_t tile_words, int64_t ntiles,
                                                      int64_t tiles_per_inst, int64_t inst_words, int mp,
-                                                     int inverse, int fold) {
+                                                     int fold) {
   extern __shared__ uint32_t sw[];
   const int n4 = (int)(tile_words >> 2);
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -249,35 +252,39 @@ __global__ __launch_bounds__(1024) void qc_conv_rows(uint32_t* __restrict__ Bw, 
 #pragma unroll
         for (int u = 0; u < CR_LD; u++) {
           const int f = f0 + u * (int)blockDim.x;
-          if (f < n4) ((uint4*)sw)[f] = v[u];
+          if (f < n4) *(uint4*)(sw + cbp_at<PAD>(4 * (int64_t)f)) = v[u];
         }
       }
     } else {
       for (int64_t i = threadIdx.x; i < tile_words; i += blockDim.x) sw[i] = g[i];
     }
     __syncthreads();
-    cbt_convert(sw, mp, inverse != 0, threadIdx.x, blockDim.x, BarrierSync());
+    if constexpr (PAD)
+      cbp_convert<true, INV>(sw, mp, threadIdx.x, blockDim.x, BarrierSync());
+    else
+      cbt_convert(sw, mp, INV, threadIdx.x, blockDim.x, BarrierSync());
     if (fold && ((tile % tiles_per_inst) & 1)) {
-      // tile_words is a power of two and a multiple of 4 on this path
-      uint32_t e[16];
-      const int ne = cb_lower_terms(mp, e);
+      // tile_words is a power of two and a multiple of 4 on this path; the
+      // terms e = 2^j (j a proper submask of mp) are enumerated in the loop
       const int64_t wm = tile_words - 1;
       uint4* g4 = (uint4*)g;
-      const uint4* s4 = (const uint4*)sw;
+      auto word = [&](int64_t i) { return sw[cbp_at<PAD>(i & wm)]; };
+      auto quad = [&](int64_t i) { return *(const uint4*)(sw + cbp_at<PAD>(i & wm)); };
       for (int f = threadIdx.x; f < (int)(tile_words >> 2); f += blockDim.x) {
         const int64_t w = 4 * (int64_t)f;
-        uint4 acc = s4[f];
-        for (int q = 0; q < ne; q++) {
-          const int64_t eo = e[q] >> 5;
-          const int eb = (int)(e[q] & 31);
+        uint4 acc = quad(w);
+        for (int j = 0; j < mp; j++) {
+          if (j & ~mp) continue;
+          const int64_t eo = ((int64_t)1 << j) >> 5;
+          const int eb = j < 5 ? 1 << j : 0;
           uint4 c4;
           if ((eo & 3) == 0) {
-            c4 = s4[((w - eo) & wm) >> 2];
+            c4 = quad(w - eo);
           } else {
-            c4 = make_uint4(sw[(w - eo) & wm], sw[(w - eo + 1) & wm], sw[(w - eo + 2) & wm], sw[(w - eo + 3) & wm]);
+            c4 = make_uint4(word(w - eo), word(w - eo + 1), word(w - eo + 2), word(w - eo + 3));
           }
           if (eb) {
-            const uint32_t pv = sw[(w - eo - 1) & wm];
+            const uint32_t pv = word(w - eo - 1);
             acc.x ^= (c4.x << eb) | (pv >> (32 - eb));
             acc.y ^= (c4.y << eb) | (c4.x >> (32 - eb));
             acc.z ^= (c4.z << eb) | (c4.y >> (32 - eb));
@@ -290,12 +297,37 @@ __global__ __launch_bounds__(1024) void qc_conv_rows(uint32_t* __restrict__ Bw, 
       }
     } else if ((tile_words & 3) == 0) {
       uint4* g4 = (uint4*)g;
-      for (int f = threadIdx.x; f < n4; f += blockDim.x) g4[f] = ((const uint4*)sw)[f];
+      for (int f = threadIdx.x; f < n4; f += blockDim.x) g4[f] = *(const uint4*)(sw + cbp_at<PAD>(4 * (int64_t)f));
     } else {
       for (int64_t i = threadIdx.x; i < tile_words; i += blockDim.x) g[i] = sw[i];
     }
     __syncthreads();
   }
+}
+
+// the blocked, padded form needs 1024-bit blocks: strings of >= 2^10 coefficients
+bool row_pad(int mp) { return mp >= 10; }
+size_t row_lds_bytes(int64_t tile_words, int mp) { return (size_t)cbp_words(tile_words, row_pad(mp)) * 4; }
+
+hipError_t launch_conv_rows(unsigned grid, hipStream_t st, uint32_t* Bw, int64_t tile_words, int64_t ntiles,
+                            int64_t tpi, int64_t inst_words, int mp, int inverse, int fold) {
+  const size_t lds = row_lds_bytes(tile_words, mp);
+#define CONV_ROWS_LAUNCH(PAD, INV)                                                                                \
+  hipLaunchKernelGGL((qc_conv_rows<PAD, INV>), dim3(grid), dim3(1024), lds, st, Bw, tile_words, ntiles, tpi, \
+                     inst_words, mp, fold)
+  if (row_pad(mp)) {
+    if (inverse)
+      CONV_ROWS_LAUNCH(true, true);
+    else
+      CONV_ROWS_LAUNCH(true, false);
+  } else {
+    if (inverse)
+      CONV_ROWS_LAUNCH(false, true);
+    else
+      CONV_ROWS_LAUNCH(false, false);
+  }
+#undef CONV_ROWS_LAUNCH
+  return hipGetLastError();
 }
 
 // ---- FFT radix pass -----------------------------------------------------------
@@ -826,9 +858,15 @@ hipError_t configure_lds() {
   e = hipFuncSetAttribute((const void*)qcs_prepare_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
                           ((int64_t)1 << PREP_LDS_MMAX) * 4);
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)qc_conv_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          ((int64_t)1 << ROW_MP_MAX) / 8);
-  if (e != hipSuccess) return e;
+  {
+    const void* fns[4] = {(const void*)qc_conv_rows<false, false>, (const void*)qc_conv_rows<false, true>,
+                          (const void*)qc_conv_rows<true, false>, (const void*)qc_conv_rows<true, true>};
+    for (int i = 0; i < 4; i++) {
+      e = hipFuncSetAttribute(fns[i], hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)row_lds_bytes(((int64_t)1 << ROW_MP_MAX) / 32, ROW_MP_MAX));
+      if (e != hipSuccess) return e;
+    }
+  }
   e = hipFuncSetAttribute((const void*)qc_conv_lds<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (1 << CONV_B) * 16);
   if (e != hipSuccess) return e;
@@ -918,9 +956,9 @@ hipError_t conv_forward(Bufs b, int64_t M, int P, int mp, hipStream_t st) {
   if (e != hipSuccess) return e;
   const int64_t tpi = (int64_t)P * 128;
   cuot_rec_begin(st, "qc_conv_rows", 2.0 * (double)tpi * b.B * row_words * 4);
-    hipLaunchKernelGGL(qc_conv_rows, dim3(capg(tpi * b.B, 4096)), dim3(1024), (size_t)row_words * 4, st, b.Bw,
-                     row_words, tpi * b.B, tpi, inst_words, mp, 0, 0);
+  e = launch_conv_rows(capg(tpi * b.B, 4096), st, b.Bw, row_words, tpi * b.B, tpi, inst_words, mp, 0, 0);
   cuot_rec_end(st);
+  if (e != hipSuccess) return e;
   return cuot_launch_transpose_planes2(b.Bw, Mp, b.X, true, b.B * P, P, inst_words, 128 * row_words, b.XS, M, st);
 }
 
@@ -936,10 +974,10 @@ hipError_t conv_inverse(Bufs b, int64_t M, int P, int mtop, bool fold, hipStream
   const int64_t tile_words = row_words / 2;
   const int64_t tpi = (int64_t)P * 128 * 2;
   cuot_rec_begin(st, "qc_conv_rows", 2.0 * (double)tpi * b.B * tile_words * 4);
-    hipLaunchKernelGGL(qc_conv_rows, dim3(capg(tpi * b.B, 8192)), dim3(1024), (size_t)tile_words * 4, st, b.Bw,
-                     tile_words, tpi * b.B, tpi, inst_words, mtop, 1, fold ? 1 : 0);
+  e = launch_conv_rows(capg(tpi * b.B, 8192), st, b.Bw, tile_words, tpi * b.B, tpi, inst_words, mtop, 1,
+                       fold ? 1 : 0);
   cuot_rec_end(st);
-  if (fold) return hipGetLastError();  // the output is read from the row layout
+  if (e != hipSuccess || fold) return e;  // folded: the output is read from the row layout
   return cuot_launch_transpose_planes2(b.Bw, M, b.X, true, b.B * P, P, inst_words, 128 * row_words, b.XS, M, st);
 }
 

@@ -106,6 +106,34 @@ int main() {
       return 1;
     }
   }
+  // blocked form (levels >= 10 on the string, 9..0 per 1024-bit block), with
+  // and without the padded storage
+  for (int mp = 10; mp <= 20; mp++)
+    for (int inv = 0; inv < 2; inv++) {
+      const int64_t n = 1ll << mp, nw = n / 32;
+      std::vector<uint32_t> W(nw);
+      for (auto& x : W) x = (uint32_t)rng();
+      std::vector<uint32_t> A = W, B = W;
+      cb_convert(A.data(), mp, inv, 0, 1, nosync);
+      if (inv)
+        cbp_convert<false, true>(B.data(), mp, 0, 1, nosync);
+      else
+        cbp_convert<false, false>(B.data(), mp, 0, 1, nosync);
+      std::vector<uint32_t> P(cbp_words(nw, true) + 4, 0xdeadbeefu);
+      for (int64_t i = 0; i < nw; i++) P[cbp_at<true>(i)] = W[i];
+      if (inv)
+        cbp_convert<true, true>(P.data(), mp, 0, 1, nosync);
+      else
+        cbp_convert<true, false>(P.data(), mp, 0, 1, nosync);
+      bool ok = A == B;
+      for (int64_t i = 0; i < nw; i++) ok = ok && P[cbp_at<true>(i)] == A[i];
+      for (int64_t i = 0; i < nw; i += 32)
+        for (int pd = 32; pd < 36; pd++) ok = ok && P[i + (i >> 5) * 4 + pd] == 0xdeadbeefu;
+      if (!ok) {
+        printf("blocked form differs mp=%d inv=%d\n", mp, inv);
+        return 1;
+      }
+    }
   printf("conversion ok\n");
   return 0;
 }

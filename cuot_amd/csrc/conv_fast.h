@@ -314,74 +314,34 @@ CB_HD cbt_w4* cbp_quad(uint32_t* w, int64_t i) {
 }
 
 template <int K, bool PAD, int J = 0>
-CB_HD void cbp_hi_terms(uint32_t& corr, const uint32_t* w, int64_t hb, int64_t hw, int64_t wi) {
+CB_HD void cbp_hi_terms(uint32_t& corr, const uint32_t* w, int base, int hb, int hw, int wi) {
   if constexpr (J < K) {
     if constexpr ((J & ~K) == 0) {
       if constexpr (J >= 5) {
-        constexpr int64_t ew = (int64_t)1 << (J - 5);
-        if (wi < ew) corr ^= w[cbp_at<PAD>(hb + wi + hw - ew)];
+        constexpr int ew = 1 << (J - 5);
+        if (wi < ew) corr ^= w[base + cbp_at<PAD>(hb + wi + hw - ew)];
       } else {
-        if (wi == 0) corr ^= w[cbp_at<PAD>(hb + hw - 1)] >> (32 - (1 << J));
+        if (wi == 0) corr ^= w[base + cbp_at<PAD>(hb + hw - 1)] >> (32 - (1 << J));
       }
     }
-    cbp_hi_terms<K, PAD, J + 1>(corr, w, hb, hw, wi);
+    cbp_hi_terms<K, PAD, J + 1>(corr, w, base, hb, hw, wi);
   }
 }
 
+// hi_fix of level K (>= 10) on the shared string, 32-bit indices
 template <int K, bool PAD>
-CB_HD void cbp_hi_fix(uint32_t* w, int64_t nw, int64_t tid, int64_t nth) {
+CB_HD void cbp_hi_fix(uint32_t* w, int base, int nw, int tid, int nth) {
   constexpr int JM = cbt_jmax<K>();
   constexpr int lhw = K - 5;
-  constexpr int64_t hw = (int64_t)1 << lhw;
-  constexpr int lnwr = JM >= 5 ? JM - 5 : 0;
-  const int64_t total = (nw >> (lhw + 1)) << lnwr;
-  for (int64_t t = tid; t < total; t += nth) {
-    const int64_t g = t >> lnwr, wi = t & (((int64_t)1 << lnwr) - 1);
-    const int64_t hb = (g << (lhw + 1)) + hw;
-    uint32_t corr = 0;
-    cbp_hi_terms<K, PAD>(corr, w, hb, hw, wi);
-    w[cbp_at<PAD>(hb + wi)] ^= corr;
-  }
-}
-
-template <int J, bool PAD>
-CB_HD void cbp_lo_term4(cbt_w4& L, const cbt_w4& H, const cbt_w4& P, const uint32_t* w, int64_t hb, int wj) {
-  if constexpr (J >= 7) {
-    constexpr int qw = 1 << (J - 5);
-    if (wj >= qw) {
-      const cbt_w4 Q = *cbp_quad<PAD>((uint32_t*)w, hb + wj - qw);
-      cbt_x4(L, Q.x, Q.y, Q.z, Q.w);
-    }
-  } else {
-    cbt_lo_term4<J>(L, H, P, nullptr, wj);  // J < 7 reads only H and P
-  }
-}
-
-template <int K, bool PAD, int J = 0>
-CB_HD void cbp_lo_terms4(cbt_w4& L, const cbt_w4& H, const cbt_w4& P, const uint32_t* w, int64_t hb, int wj) {
-  if constexpr (J < K) {
-    if constexpr ((J & ~K) == 0) cbp_lo_term4<J, PAD>(L, H, P, w, hb, wj);
-    cbp_lo_terms4<K, PAD, J + 1>(L, H, P, w, hb, wj);
-  }
-}
-
-template <int K, bool PAD>
-CB_HD void cbp_lo_update4(uint32_t* w, int64_t nw, int64_t tid, int64_t nth) {
-  static_assert(K >= 7, "four-word form needs >= 4 words per half");
-  constexpr int lhw = K - 5;
   constexpr int hw = 1 << lhw;
-  const int64_t total4 = nw >> 3;
-  for (int64_t t = tid; t < total4; t += nth) {
-    const int64_t t4 = t << 2;
-    const int64_t g = t4 >> lhw;
-    const int wj = (int)(t4 & (hw - 1));
-    const int64_t lb = g << (lhw + 1), hb = lb + hw;
-    cbt_w4 L = *cbp_quad<PAD>(w, lb + wj);
-    const cbt_w4 H = *cbp_quad<PAD>(w, hb + wj);
-    cbt_w4 P = {0u, 0u, 0u, 0u};
-    if (wj >= 4) P = *cbp_quad<PAD>(w, hb + wj - 4);
-    cbp_lo_terms4<K, PAD>(L, H, P, w, hb, wj);
-    *cbp_quad<PAD>(w, lb + wj) = L;
+  constexpr int lnwr = JM >= 5 ? JM - 5 : 0;
+  const int total = (nw >> (lhw + 1)) << lnwr;
+  for (int t = tid; t < total; t += nth) {
+    const int g = t >> lnwr, wi = t & ((1 << lnwr) - 1);
+    const int hb = (g << (lhw + 1)) + hw;
+    uint32_t corr = 0;
+    cbp_hi_terms<K, PAD>(corr, w, base, hb, hw, wi);
+    w[base + cbp_at<PAD>(hb + wi)] ^= corr;
   }
 }
 
@@ -391,19 +351,19 @@ CB_HD void cbp_lo_update4(uint32_t* w, int64_t nw, int64_t tid, int64_t nth) {
 // below 16 words (J <= 8) are formed in registers, and those with J >= 9
 // read one quad each, under a condition that is the same for the whole run.
 template <int K, bool PAD, int J = 9>
-CB_HD void cbq_far_terms(cbt_w4* L, const uint32_t* w, int hb, int wj) {
+CB_HD void cbq_far_terms(cbt_w4* L, const uint32_t* w, int base, int hb, int wj) {
   if constexpr (J < K) {
     if constexpr ((J & ~K) == 0) {
       constexpr int qw = 1 << (J - 5);
       if (wj >= qw) {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-          const cbt_w4 Q = *(const cbt_w4*)(w + cbp_at<PAD>(hb + wj + 4 * i - qw));
+          const cbt_w4 Q = *(const cbt_w4*)(w + base + cbp_at<PAD>(hb + wj + 4 * i - qw));
           cbt_x4(L[i], Q.x, Q.y, Q.z, Q.w);
         }
       }
     }
-    cbq_far_terms<K, PAD, J + 1>(L, w, hb, wj);
+    cbq_far_terms<K, PAD, J + 1>(L, w, base, hb, wj);
   }
 }
 
@@ -423,7 +383,7 @@ CB_HD void cbq_near_terms(cbt_w4& L, const cbt_w4& H, const cbt_w4& P, const cbt
 }
 
 template <int K, bool PAD>
-CB_HD void cbq_lo_update(uint32_t* w, int nw, int tid, int nth) {
+CB_HD void cbq_lo_update(uint32_t* w, int base, int nw, int tid, int nth) {
   static_assert(K >= 10, "runs of 16 words need >= 32 words per half");
   constexpr int lhw = K - 5;
   constexpr int hw = 1 << lhw;
@@ -435,18 +395,18 @@ CB_HD void cbq_lo_update(uint32_t* w, int nw, int tid, int nth) {
     cbt_w4 L[4], H[6];  // H[0], H[1]: hi quads 8 and 4 words before the run (zero at the half's start)
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      L[i] = *(const cbt_w4*)(w + cbp_at<PAD>(lb + wj + 4 * i));
-      H[i + 2] = *(const cbt_w4*)(w + cbp_at<PAD>(hb + wj + 4 * i));
+      L[i] = *(const cbt_w4*)(w + base + cbp_at<PAD>(lb + wj + 4 * i));
+      H[i + 2] = *(const cbt_w4*)(w + base + cbp_at<PAD>(hb + wj + 4 * i));
     }
     const int pb = wj > 0 ? hb + wj - 8 : hb;  // an in-range address either way
-    H[0] = *(const cbt_w4*)(w + cbp_at<PAD>(pb));
-    H[1] = *(const cbt_w4*)(w + cbp_at<PAD>(pb + 4));
+    H[0] = *(const cbt_w4*)(w + base + cbp_at<PAD>(pb));
+    H[1] = *(const cbt_w4*)(w + base + cbp_at<PAD>(pb + 4));
     if (wj == 0) H[0] = H[1] = cbt_w4{0u, 0u, 0u, 0u};
-    cbq_far_terms<K, PAD>(L, w, hb, wj);
+    cbq_far_terms<K, PAD>(L, w, base, hb, wj);
 #pragma unroll
     for (int i = 0; i < 4; i++) cbq_near_terms<K>(L[i], H[i + 2], H[i + 1], H[i]);
 #pragma unroll
-    for (int i = 0; i < 4; i++) *(cbt_w4*)(w + cbp_at<PAD>(lb + wj + 4 * i)) = L[i];
+    for (int i = 0; i < 4; i++) *(cbt_w4*)(w + base + cbp_at<PAD>(lb + wj + 4 * i)) = L[i];
   }
 }
 
@@ -486,8 +446,11 @@ CB_HD void cbr_level(uint32_t* W) {
   }
 }
 
-// levels 9..0 (forward) or 0..9 (inverse) of one 1024-bit block
-template <bool INV>
+// levels 9..0 (forward) or 0..9 (inverse) of one 1024-bit block; with
+// INWORD = false the in-word levels 4..0 are left to the caller (they are the
+// first inverse and the last forward step, so they can ride on the copy
+// between global memory and LDS)
+template <bool INV, bool INWORD = true>
 CB_HD void cbr_block(uint32_t* W) {
   if constexpr (!INV) {
     cbr_level<9, false>(W);
@@ -495,17 +458,21 @@ CB_HD void cbr_block(uint32_t* W) {
     cbr_level<7, false>(W);
     cbr_level<6, false>(W);
     cbr_level<5, false>(W);
+    if constexpr (INWORD) {
 #pragma unroll
-    for (int i = 0; i < 32; i++) W[i] = cbt_word_all(W[i], false);
+      for (int i = 0; i < 32; i++) W[i] = cbt_word_all(W[i], false);
+    }
   } else {
+    if constexpr (INWORD) {
 #pragma unroll
-    for (int i = 0; i < 32; i++) {
-      W[i] = cbt_word_all(W[i], true);
+      for (int i = 0; i < 32; i++) {
+        W[i] = cbt_word_all(W[i], true);
 #ifdef __HIP_DEVICE_COMPILE__
-      // one word at a time: interleaving all 32 in-word transforms would hold
-      // more registers than the whole block form needs otherwise
-      __builtin_amdgcn_sched_barrier(0);
+        // one word at a time: interleaving all 32 in-word transforms would
+        // hold more registers than the whole block form needs otherwise
+        __builtin_amdgcn_sched_barrier(0);
 #endif
+      }
     }
     cbr_level<5, true>(W);
     cbr_level<6, true>(W);
@@ -515,7 +482,7 @@ CB_HD void cbr_block(uint32_t* W) {
   }
 }
 
-template <bool PAD, bool INV>
+template <bool PAD, bool INV, bool INWORD = true>
 CB_HD void cbp_blocks(uint32_t* w, int64_t nw, int64_t tid, int64_t nth) {
   const int64_t nblk = nw >> 5;
   for (int64_t b = tid; b < nblk; b += nth) {
@@ -528,26 +495,30 @@ CB_HD void cbp_blocks(uint32_t* w, int64_t nw, int64_t tid, int64_t nth) {
       W[4 * q + 2] = v.z;
       W[4 * q + 3] = v.w;
     }
-    cbr_block<INV>(W);
+    cbr_block<INV, INWORD>(W);
 #pragma unroll
     for (int q = 0; q < 8; q++)
       *cbp_quad<PAD>(w, 32 * b + 4 * q) = cbt_w4{W[4 * q], W[4 * q + 1], W[4 * q + 2], W[4 * q + 3]};
   }
 }
 
+// The dispatcher stays out of line.  Its pointer argument is proved an LDS
+// address (so that the accesses are LDS instructions, not flat ones) only
+// while every caller passes the workgroup's LDS base itself; a string at an
+// offset within it is therefore given as w plus a word offset `base`.
 template <bool PAD, typename Sync>
-CB_HD void cbp_level(uint32_t* w, int64_t nw, int k, bool inverse, int64_t tid, int64_t nth, Sync sync) {
-#define CBP_CASE(K)                                        \
-  case K:                                                  \
-    if (!inverse) {                                        \
-      cbp_hi_fix<K, PAD>(w, nw, tid, nth);                 \
-      sync();                                              \
-      cbq_lo_update<K, PAD>(w, (int)nw, (int)tid, (int)nth); \
-    } else {                                               \
-      cbq_lo_update<K, PAD>(w, (int)nw, (int)tid, (int)nth); \
-      sync();                                              \
-      cbp_hi_fix<K, PAD>(w, nw, tid, nth);                 \
-    }                                                      \
+CB_HD void cbp_level(uint32_t* w, int base, int64_t nw, int k, bool inverse, int64_t tid, int64_t nth, Sync sync) {
+#define CBP_CASE(K)                                                  \
+  case K:                                                            \
+    if (!inverse) {                                                  \
+      cbp_hi_fix<K, PAD>(w, base, (int)nw, (int)tid, (int)nth);      \
+      sync();                                                        \
+      cbq_lo_update<K, PAD>(w, base, (int)nw, (int)tid, (int)nth);   \
+    } else {                                                         \
+      cbq_lo_update<K, PAD>(w, base, (int)nw, (int)tid, (int)nth);   \
+      sync();                                                        \
+      cbp_hi_fix<K, PAD>(w, base, (int)nw, (int)tid, (int)nth);      \
+    }                                                                \
     break;
   switch (k) {
     CBP_CASE(10)
@@ -568,19 +539,21 @@ CB_HD void cbp_level(uint32_t* w, int64_t nw, int k, bool inverse, int64_t tid, 
 }
 
 // whole conversion of a 2^mp-coefficient string, 10 <= mp <= 20, stored with
-// or without the padding (same results as cbt_convert).  The direction is a
-// template argument: one kernel holding both block forms would need more
-// registers than either.
-template <bool PAD, bool INV, typename Sync>
-CB_HD void cbp_convert(uint32_t* w, int mp, int64_t tid, int64_t nth, Sync sync) {
+// or without the padding at w + base (same results as cbt_convert; with
+// INWORD = false the caller applies cbt_word_all to every word before an
+// inverse or after a forward conversion).  The direction is a template
+// argument: one kernel holding both block forms would need more registers
+// than either.
+template <bool PAD, bool INV, bool INWORD = true, typename Sync>
+CB_HD void cbp_convert(uint32_t* w, int base, int mp, int64_t tid, int64_t nth, Sync sync) {
   const int64_t nw = ((int64_t)1 << mp) >> 5;
   if constexpr (!INV) {
-    for (int k = mp - 1; k >= 10; k--) cbp_level<PAD>(w, nw, k, false, tid, nth, sync);
-    cbp_blocks<PAD, false>(w, nw, tid, nth);
+    for (int k = mp - 1; k >= 10; k--) cbp_level<PAD>(w, base, nw, k, false, tid, nth, sync);
+    cbp_blocks<PAD, false, INWORD>(w + base, nw, tid, nth);
     sync();
   } else {
-    cbp_blocks<PAD, true>(w, nw, tid, nth);
+    cbp_blocks<PAD, true, INWORD>(w + base, nw, tid, nth);
     sync();
-    for (int k = 10; k <= mp - 1; k++) cbp_level<PAD>(w, nw, k, true, tid, nth, sync);
+    for (int k = 10; k <= mp - 1; k++) cbp_level<PAD>(w, base, nw, k, true, tid, nth, sync);
   }
 }

@@ -230,101 +230,174 @@ struct BarrierSync {
 // folded at half its length (s_mp = X^(2^mp) + sum_E X^e).
 // PAD (strings of >= 2^10 coefficients): the string sits in LDS with four pad
 // words after every 32, and levels 9..0 run per 1024-bit block in registers.
-template <bool PAD, bool INV>
+// MULTI (PAD, strings shorter than 2^20): a workgroup converts S = 1024 /
+// (blocks per string) strings side by side, one lane per block, so that short
+// strings still occupy every lane (the same LDS footprint for every length:
+// 1024 padded blocks).  Its per-lane LDS base costs registers, so there the
+// in-word levels (the first inverse and the last forward step) ride on the
+// copies between global memory and LDS (INW); a single string keeps them in
+// the register blocks, which measured faster.
+CUOT_DEV uint4 inword4(uint4 v, bool inv) {
+  v.x = cbt_word_all(v.x, inv);
+  v.y = cbt_word_all(v.y, inv);
+  v.z = cbt_word_all(v.z, inv);
+  v.w = cbt_word_all(v.w, inv);
+  return v;
+}
+
+template <bool PAD, bool INW>
+__device__ __forceinline__ void rows_load(const uint32_t* g, uint32_t* sw, int64_t tile_words, int lt, int tps) {
+  const int n4 = (int)(tile_words >> 2);
+  if ((tile_words & 3) == 0) {
+    // 16-byte accesses, CR_LD in flight per lane
+    const uint4* g4 = (const uint4*)g;
+    for (int f0 = lt; f0 < n4; f0 += tps * CR_LD) {
+      uint4 v[CR_LD];
+#pragma unroll
+      for (int u = 0; u < CR_LD; u++) {
+        const int f = f0 + u * tps;
+        v[u] = z4();
+        if (f < n4) v[u] = g4[f];
+      }
+#pragma unroll
+      for (int u = 0; u < CR_LD; u++) {
+        const int f = f0 + u * tps;
+        if (f < n4) *(uint4*)(sw + cbp_at<PAD>(4 * (int64_t)f)) = INW ? inword4(v[u], true) : v[u];
+      }
+    }
+  } else {
+    for (int64_t i = lt; i < tile_words; i += tps) sw[i] = g[i];
+  }
+}
+
+// fold (inverse, odd tiles): the terms e = 2^j (j a proper submask of mp) are
+// enumerated in the loop; tile_words is a power of two and a multiple of 4
+template <bool PAD, bool INW>
+__device__ __forceinline__ void rows_store(uint32_t* g, const uint32_t* sw, int64_t tile_words, int mp, bool fold_q,
+                                           int lt, int tps) {
+  const int n4 = (int)(tile_words >> 2);
+  if (fold_q) {
+    const int64_t wm = tile_words - 1;
+    uint4* g4 = (uint4*)g;
+    auto word = [&](int64_t i) { return sw[cbp_at<PAD>(i & wm)]; };
+    auto quad = [&](int64_t i) { return *(const uint4*)(sw + cbp_at<PAD>(i & wm)); };
+    for (int f = lt; f < n4; f += tps) {
+      const int64_t w = 4 * (int64_t)f;
+      uint4 acc = quad(w);
+      for (int j = 0; j < mp; j++) {
+        if (j & ~mp) continue;
+        const int64_t eo = ((int64_t)1 << j) >> 5;
+        const int eb = j < 5 ? 1 << j : 0;
+        uint4 c4;
+        if ((eo & 3) == 0) {
+          c4 = quad(w - eo);
+        } else {
+          c4 = make_uint4(word(w - eo), word(w - eo + 1), word(w - eo + 2), word(w - eo + 3));
+        }
+        if (eb) {
+          const uint32_t pv = word(w - eo - 1);
+          acc.x ^= (c4.x << eb) | (pv >> (32 - eb));
+          acc.y ^= (c4.y << eb) | (c4.x >> (32 - eb));
+          acc.z ^= (c4.z << eb) | (c4.y >> (32 - eb));
+          acc.w ^= (c4.w << eb) | (c4.z >> (32 - eb));
+        } else {
+          acc = x4(acc, c4);
+        }
+      }
+      g4[f] = acc;
+    }
+  } else if ((tile_words & 3) == 0) {
+    uint4* g4 = (uint4*)g;
+    for (int f = lt; f < n4; f += tps) {
+      const uint4 v = *(const uint4*)(sw + cbp_at<PAD>(4 * (int64_t)f));
+      g4[f] = INW ? inword4(v, false) : v;
+    }
+  } else {
+    for (int64_t i = lt; i < tile_words; i += tps) g[i] = sw[i];
+  }
+}
+
+template <bool PAD, bool INV, bool MULTI>
 __global__ __launch_bounds__(1024) void qc_conv_rows(uint32_t* __restrict__ Bw, int64_t tile_words, int64_t ntiles,
                                                      int64_t tiles_per_inst, int64_t inst_words, int mp,
                                                      int fold) {
-  extern __shared__ uint32_t sw[];
-  const int n4 = (int)(tile_words >> 2);
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    uint32_t* g = Bw + (tile / tiles_per_inst) * inst_words + (tile % tiles_per_inst) * tile_words;
-    if ((tile_words & 3) == 0) {
-      // 16-byte accesses, CR_LD in flight per lane
-      const uint4* g4 = (const uint4*)g;
-      for (int f0 = threadIdx.x; f0 < n4; f0 += blockDim.x * CR_LD) {
-        uint4 v[CR_LD];
-#pragma unroll
-        for (int u = 0; u < CR_LD; u++) {
-          const int f = f0 + u * (int)blockDim.x;
-          v[u] = z4();
-          if (f < n4) v[u] = g4[f];
-        }
-#pragma unroll
-        for (int u = 0; u < CR_LD; u++) {
-          const int f = f0 + u * (int)blockDim.x;
-          if (f < n4) *(uint4*)(sw + cbp_at<PAD>(4 * (int64_t)f)) = v[u];
-        }
-      }
-    } else {
-      for (int64_t i = threadIdx.x; i < tile_words; i += blockDim.x) sw[i] = g[i];
+  static_assert(PAD || !MULTI, "several strings per workgroup need the blocked form");
+  extern __shared__ uint32_t sw_all[];
+  if constexpr (!MULTI) {
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+      uint32_t* g = Bw + (tile / tiles_per_inst) * inst_words + (tile % tiles_per_inst) * tile_words;
+      rows_load<PAD, false>(g, sw_all, tile_words, threadIdx.x, blockDim.x);
+      __syncthreads();
+      if constexpr (PAD)
+        cbp_convert<true, INV>(sw_all, 0, mp, threadIdx.x, blockDim.x, BarrierSync());
+      else
+        cbt_convert(sw_all, mp, INV, threadIdx.x, blockDim.x, BarrierSync());
+      rows_store<PAD, false>(g, sw_all, tile_words, mp, fold && ((tile % tiles_per_inst) & 1), threadIdx.x,
+                             blockDim.x);
+      __syncthreads();
     }
-    __syncthreads();
-    if constexpr (PAD)
-      cbp_convert<true, INV>(sw, mp, threadIdx.x, blockDim.x, BarrierSync());
-    else
-      cbt_convert(sw, mp, INV, threadIdx.x, blockDim.x, BarrierSync());
-    if (fold && ((tile % tiles_per_inst) & 1)) {
-      // tile_words is a power of two and a multiple of 4 on this path; the
-      // terms e = 2^j (j a proper submask of mp) are enumerated in the loop
-      const int64_t wm = tile_words - 1;
-      uint4* g4 = (uint4*)g;
-      auto word = [&](int64_t i) { return sw[cbp_at<PAD>(i & wm)]; };
-      auto quad = [&](int64_t i) { return *(const uint4*)(sw + cbp_at<PAD>(i & wm)); };
-      for (int f = threadIdx.x; f < (int)(tile_words >> 2); f += blockDim.x) {
-        const int64_t w = 4 * (int64_t)f;
-        uint4 acc = quad(w);
-        for (int j = 0; j < mp; j++) {
-          if (j & ~mp) continue;
-          const int64_t eo = ((int64_t)1 << j) >> 5;
-          const int eb = j < 5 ? 1 << j : 0;
-          uint4 c4;
-          if ((eo & 3) == 0) {
-            c4 = quad(w - eo);
-          } else {
-            c4 = make_uint4(word(w - eo), word(w - eo + 1), word(w - eo + 2), word(w - eo + 3));
-          }
-          if (eb) {
-            const uint32_t pv = word(w - eo - 1);
-            acc.x ^= (c4.x << eb) | (pv >> (32 - eb));
-            acc.y ^= (c4.y << eb) | (c4.x >> (32 - eb));
-            acc.z ^= (c4.z << eb) | (c4.y >> (32 - eb));
-            acc.w ^= (c4.w << eb) | (c4.z >> (32 - eb));
-          } else {
-            acc = x4(acc, c4);
-          }
-        }
-        g4[f] = acc;
-      }
-    } else if ((tile_words & 3) == 0) {
-      uint4* g4 = (uint4*)g;
-      for (int f = threadIdx.x; f < n4; f += blockDim.x) g4[f] = *(const uint4*)(sw + cbp_at<PAD>(4 * (int64_t)f));
-    } else {
-      for (int64_t i = threadIdx.x; i < tile_words; i += blockDim.x) g[i] = sw[i];
+  } else {
+    const int lgt = mp - 10;  // log2 lanes per string: one lane per 1024-bit block
+    const int tps = 1 << lgt;
+    const int S = (int)blockDim.x >> lgt;  // strings per workgroup
+    const int lt = (int)threadIdx.x & (tps - 1);
+    const int sl = (int)threadIdx.x >> lgt;
+    const int base = sl * (int)cbp_words(tile_words, true);  // this lane's string in LDS
+    uint32_t* sw = sw_all + base;
+    // 32-bit tile arithmetic (tile counts stay far below 2^31); the global
+    // pointer is recomputed after the conversion rather than held through it
+    const unsigned tpi = (unsigned)tiles_per_inst;
+    for (unsigned t0 = blockIdx.x * (unsigned)S; t0 < (unsigned)ntiles; t0 += gridDim.x * (unsigned)S) {
+      // every string group runs the barriers of the conversion; one past the
+      // last tile converts its own LDS region and stores nothing
+      const unsigned tile = t0 + (unsigned)sl;
+      const bool live = tile < (unsigned)ntiles;
+      if (live)
+        rows_load<true, INV>(Bw + (int64_t)(tile / tpi) * inst_words + (int64_t)(tile % tpi) * tile_words, sw,
+                             tile_words, lt, tps);
+      __syncthreads();
+      cbp_convert<true, INV, false>(sw_all, base, mp, lt, tps, BarrierSync());
+      if (live)
+        rows_store<true, !INV>(Bw + (int64_t)(tile / tpi) * inst_words + (int64_t)(tile % tpi) * tile_words, sw,
+                               tile_words, mp, fold && ((tile % tpi) & 1), lt, tps);
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
 // the blocked, padded form needs 1024-bit blocks: strings of >= 2^10 coefficients
 bool row_pad(int mp) { return mp >= 10; }
-size_t row_lds_bytes(int64_t tile_words, int mp) { return (size_t)cbp_words(tile_words, row_pad(mp)) * 4; }
+// LDS of one workgroup: one string, or 1024 / (blocks per string) padded strings
+size_t row_lds_bytes(int64_t tile_words, int mp) {
+  return row_pad(mp) ? (size_t)cbp_words(32 * 1024, true) * 4 : (size_t)tile_words * 4;
+}
 
-hipError_t launch_conv_rows(unsigned grid, hipStream_t st, uint32_t* Bw, int64_t tile_words, int64_t ntiles,
+hipError_t launch_conv_rows(unsigned cap, hipStream_t st, uint32_t* Bw, int64_t tile_words, int64_t ntiles,
                             int64_t tpi, int64_t inst_words, int mp, int inverse, int fold) {
+  if (row_pad(mp) && (tile_words < 32 || tile_words > 32 * 1024)) return hipErrorInvalidValue;
   const size_t lds = row_lds_bytes(tile_words, mp);
-#define CONV_ROWS_LAUNCH(PAD, INV)                                                                                \
-  hipLaunchKernelGGL((qc_conv_rows<PAD, INV>), dim3(grid), dim3(1024), lds, st, Bw, tile_words, ntiles, tpi, \
-                     inst_words, mp, fold)
-  if (row_pad(mp)) {
+  const bool multi = row_pad(mp) && tile_words < 32 * 1024;
+  const int64_t per_wg = multi ? 1024 / (tile_words >> 5) : 1;  // strings per workgroup
+  const int64_t need = (ntiles + per_wg - 1) / per_wg;
+  const unsigned grid = (unsigned)(need < (int64_t)cap ? (need < 1 ? 1 : need) : cap);
+#define CONV_ROWS_LAUNCH(PAD, INV, MULTI)                                                                    \
+  hipLaunchKernelGGL((qc_conv_rows<PAD, INV, MULTI>), dim3(grid), dim3(1024), lds, st, Bw, tile_words, ntiles, \
+                     tpi, inst_words, mp, fold)
+  if (multi) {
     if (inverse)
-      CONV_ROWS_LAUNCH(true, true);
+      CONV_ROWS_LAUNCH(true, true, true);
     else
-      CONV_ROWS_LAUNCH(true, false);
+      CONV_ROWS_LAUNCH(true, false, true);
+  } else if (row_pad(mp)) {
+    if (inverse)
+      CONV_ROWS_LAUNCH(true, true, false);
+    else
+      CONV_ROWS_LAUNCH(true, false, false);
   } else {
     if (inverse)
-      CONV_ROWS_LAUNCH(false, true);
+      CONV_ROWS_LAUNCH(false, true, false);
     else
-      CONV_ROWS_LAUNCH(false, false);
+      CONV_ROWS_LAUNCH(false, false, false);
   }
 #undef CONV_ROWS_LAUNCH
   return hipGetLastError();
@@ -859,9 +932,13 @@ hipError_t configure_lds() {
                           ((int64_t)1 << PREP_LDS_MMAX) * 4);
   if (e != hipSuccess) return e;
   {
-    const void* fns[4] = {(const void*)qc_conv_rows<false, false>, (const void*)qc_conv_rows<false, true>,
-                          (const void*)qc_conv_rows<true, false>, (const void*)qc_conv_rows<true, true>};
-    for (int i = 0; i < 4; i++) {
+    const void* fns[6] = {(const void*)qc_conv_rows<false, false, false>,
+                          (const void*)qc_conv_rows<false, true, false>,
+                          (const void*)qc_conv_rows<true, false, false>,
+                          (const void*)qc_conv_rows<true, true, false>,
+                          (const void*)qc_conv_rows<true, false, true>,
+                          (const void*)qc_conv_rows<true, true, true>};
+    for (int i = 0; i < 6; i++) {
       e = hipFuncSetAttribute(fns[i], hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)row_lds_bytes(((int64_t)1 << ROW_MP_MAX) / 32, ROW_MP_MAX));
       if (e != hipSuccess) return e;
@@ -956,7 +1033,7 @@ hipError_t conv_forward(Bufs b, int64_t M, int P, int mp, hipStream_t st) {
   if (e != hipSuccess) return e;
   const int64_t tpi = (int64_t)P * 128;
   cuot_rec_begin(st, "qc_conv_rows", 2.0 * (double)tpi * b.B * row_words * 4);
-  e = launch_conv_rows(capg(tpi * b.B, 4096), st, b.Bw, row_words, tpi * b.B, tpi, inst_words, mp, 0, 0);
+  e = launch_conv_rows(4096, st, b.Bw, row_words, tpi * b.B, tpi, inst_words, mp, 0, 0);
   cuot_rec_end(st);
   if (e != hipSuccess) return e;
   return cuot_launch_transpose_planes2(b.Bw, Mp, b.X, true, b.B * P, P, inst_words, 128 * row_words, b.XS, M, st);
@@ -974,8 +1051,7 @@ hipError_t conv_inverse(Bufs b, int64_t M, int P, int mtop, bool fold, hipStream
   const int64_t tile_words = row_words / 2;
   const int64_t tpi = (int64_t)P * 128 * 2;
   cuot_rec_begin(st, "qc_conv_rows", 2.0 * (double)tpi * b.B * tile_words * 4);
-  e = launch_conv_rows(capg(tpi * b.B, 8192), st, b.Bw, tile_words, tpi * b.B, tpi, inst_words, mtop, 1,
-                       fold ? 1 : 0);
+  e = launch_conv_rows(8192, st, b.Bw, tile_words, tpi * b.B, tpi, inst_words, mtop, 1, fold ? 1 : 0);
   cuot_rec_end(st);
   if (e != hipSuccess || fold) return e;  // folded: the output is read from the row layout
   return cuot_launch_transpose_planes2(b.Bw, M, b.X, true, b.B * P, P, inst_words, 128 * row_words, b.XS, M, st);

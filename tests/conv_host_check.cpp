@@ -116,16 +116,25 @@ int main() {
       std::vector<uint32_t> A = W, B = W;
       cb_convert(A.data(), mp, inv, 0, 1, nosync);
       if (inv)
-        cbp_convert<false, true>(B.data(), mp, 0, 1, nosync);
+        cbp_convert<false, true>(B.data(), 0, mp, 0, 1, nosync);
       else
-        cbp_convert<false, false>(B.data(), mp, 0, 1, nosync);
+        cbp_convert<false, false>(B.data(), 0, mp, 0, 1, nosync);
       std::vector<uint32_t> P(cbp_words(nw, true) + 4, 0xdeadbeefu);
       for (int64_t i = 0; i < nw; i++) P[cbp_at<true>(i)] = W[i];
       if (inv)
-        cbp_convert<true, true>(P.data(), mp, 0, 1, nosync);
+        cbp_convert<true, true>(P.data() + 4, -4, mp, 0, 1, nosync);
       else
-        cbp_convert<true, false>(P.data(), mp, 0, 1, nosync);
-      bool ok = A == B;
+        cbp_convert<true, false>(P.data() + 4, -4, mp, 0, 1, nosync);
+      // in-word levels outside the blocked form
+      std::vector<uint32_t> C = W;
+      if (inv) {
+        for (auto& x : C) x = cbt_word_all(x, true);
+        cbp_convert<false, true, false>(C.data(), 0, mp, 0, 1, nosync);
+      } else {
+        cbp_convert<false, false, false>(C.data(), 0, mp, 0, 1, nosync);
+        for (auto& x : C) x = cbt_word_all(x, false);
+      }
+      bool ok = A == B && A == C;
       for (int64_t i = 0; i < nw; i++) ok = ok && P[cbp_at<true>(i)] == A[i];
       for (int64_t i = 0; i < nw; i += 32)
         for (int pd = 32; pd < 36; pd++) ok = ok && P[i + (i >> 5) * 4 + pd] == 0xdeadbeefu;

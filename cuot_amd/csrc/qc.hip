@@ -40,6 +40,15 @@ constexpr int CONV_G = 4;      // planes per conversion group (ping-pong buffer 
 constexpr int FFT_TC = 128;    // elements per FFT LDS tile (2^R * C): two tiles fit one CU's LDS
 constexpr int FFT_LOGTC = 7;
 constexpr int FFT_STRIDE = 132;  // LDS words per element (32 planes x 4 words + pad)
+#ifndef FFT_SWZ
+#define FFT_SWZ 1
+#endif
+// LDS slot of tile element e.  With the 132-word stride an element's word w of
+// plane p sits in bank group e mod 8; the butterflies of the lowest tile layers
+// (s = 0..2) touch elements whose indices share their residues mod 8 four at a
+// time.  Reflecting the residue of every odd group of eight (e ^ 7) spreads
+// each such access over all eight bank groups.
+CUOT_DEV int fslot(int e) { return FFT_SWZ ? e ^ (((e >> 3) & 1) * 7) : e; }
 constexpr int FFT_THREADS = 256;  // one (butterfly, word) task per lane per layer
 constexpr int FFT_LD = 16;  // 16-byte loads in flight per lane (tile staging)
 
@@ -431,8 +440,8 @@ __device__ __forceinline__ void fft_copy_top(int R, int C, int lgC, uint32_t* ld
     const int w = tau & 3;
     const int c = (tau >> 2) & (C - 1);
     const int bp = (tau >> 2) >> lgC;  // < 2^s: tu = bp, tv = bp + 2^s
-    const uint32_t* pu = lds + (bp * C + c) * FFT_STRIDE + w;
-    uint32_t* pv = lds + ((bp | (1 << s)) * C + c) * FFT_STRIDE + w;
+    const uint32_t* pu = lds + fslot(bp * C + c) * FFT_STRIDE + w;
+    uint32_t* pv = lds + fslot((bp | (1 << s)) * C + c) * FFT_STRIDE + w;
 #pragma unroll
     for (int p = 0; p < 32; p++) pv[4 * p] = pu[4 * p];
   }
@@ -453,8 +462,8 @@ __device__ __forceinline__ void fft_layer(int lo, int R, int C, int lgC, uint32_
     const int tv = tu | (1 << s);
     const int64_t idx = (hib << (lo + R)) | ((int64_t)tu << lo) | (cb * C + c);
     const uint64_t g = (uint64_t)idx >> (k + 1);
-    uint32_t* pu = lds + (tu * C + c) * FFT_STRIDE + w;
-    uint32_t* pv = lds + (tv * C + c) * FFT_STRIDE + w;
+    uint32_t* pu = lds + fslot(tu * C + c) * FFT_STRIDE + w;
+    uint32_t* pv = lds + fslot(tv * C + c) * FFT_STRIDE + w;
     // forward: u += c*v, v += u; inverse: v += u, u += c*v.  Only v (or the
     // new v) and the product are held in registers; u streams through LDS.
     uint32_t xv[32], acc[32];
@@ -500,7 +509,7 @@ __device__ __forceinline__ void fft_pointwise(int lo, int R, int C, int lgC,
     const int el = tau >> 2;  // = t*C + c
     const int t = el >> lgC, c = el & (C - 1);
     const int64_t idx = (hib << (lo + R)) | ((int64_t)t << lo) | (cb * C + c);
-    uint32_t* pe = lds + el * FFT_STRIDE + w;
+    uint32_t* pe = lds + fslot(el) * FFT_STRIDE + w;
     uint32_t x[32], y[32];
 #pragma unroll
     for (int p = 0; p < 32; p++) x[p] = pe[4 * p];
@@ -569,7 +578,7 @@ __global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(2))
         const int c = f & (a.C - 1);
         const int pt = f >> a.lgC;
         const int t = pt & (T - 1), p = pt >> a.R;
-        *(uint4*)&lds[(t * a.C + c) * FFT_STRIDE + p * 4] = v[u];
+        *(uint4*)&lds[fslot(t * a.C + c) * FFT_STRIDE + p * 4] = v[u];
       }
     }
     __syncthreads();
@@ -601,7 +610,7 @@ __global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(2))
         const int c = f & (a.C - 1);
         const int pt = f >> a.lgC;
         const int t = pt & (T - 1);
-        w[u] = *(const uint4*)&lds[(t * a.C + c) * FFT_STRIDE + (pt >> a.R) * 4];
+        w[u] = *(const uint4*)&lds[fslot(t * a.C + c) * FFT_STRIDE + (pt >> a.R) * 4];
       }
     }
     if (tile + G < ntiles) load_tile(tile + G);
@@ -764,6 +773,78 @@ __global__ __launch_bounds__(1024) void qcs_prepare_lds(const QcPrep* __restrict
     __syncthreads();
   }
   for (int64_t c = tid; c < M; c += nth) q.ahat[c] = L[c];
+}
+
+// Tiled forms for long polynomials: the conversion levels below QCS_LB and the
+// FFT layers run in LDS tiles of 2^QCS_LB words, one launch each instead of
+// one per level / layer (the upper conversion levels keep their launches).
+constexpr int QCS_LB = 12;
+
+// conversion levels kmax..0 of the lower half on consecutive tiles
+__global__ __launch_bounds__(1024) void qcs_conv_tile(const QcPrep* __restrict__ pr, QcPrep pr1, int kmax) {
+  __shared__ uint32_t L[1 << QCS_LB];
+  constexpr int TW = 1 << QCS_LB;
+  uint32_t* A = (pr ? pr[blockIdx.y] : pr1).ahat + ((int64_t)blockIdx.x << QCS_LB);
+  for (int i = threadIdx.x; i < TW; i += blockDim.x) L[i] = A[i];
+  __syncthreads();
+  for (int k = kmax; k >= 0; k--) {
+    uint32_t e[16];
+    const int ne = lower_terms(k, e);
+    if (ne == 0) continue;
+    const int half = 1 << k;
+    const int emax = (int)e[ne - 1];
+    const int le = __builtin_ctz((uint32_t)emax);
+    for (int t = threadIdx.x; t < ((TW >> (k + 1)) << le); t += blockDim.x) {
+      const int g = t >> le, i = t & (emax - 1);
+      uint32_t* h = L + g * 2 * half + half;
+      uint32_t v = h[i];
+      for (int r = 0; r < ne; r++)
+        if ((int)e[r] > i) v ^= h[i + half - (int)e[r]];
+      h[i] = v;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < (TW >> 1); t += blockDim.x) {
+      const int g = t >> k, j = t & (half - 1);
+      uint32_t* l = L + g * 2 * half;
+      uint32_t v = l[j];
+      for (int r = 0; r < ne; r++) {
+        if ((int)e[r] > j) break;
+        v ^= l[half + j - (int)e[r]];
+      }
+      l[j] = v;
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < TW; i += blockDim.x) A[i] = L[i];
+}
+
+// FFT layers khi..lo on tiles of 2^R elements spaced 2^lo apart times 2^lgC
+// consecutive columns (2^(R + lgC) = 2^QCS_LB words, lgC <= lo)
+__global__ __launch_bounds__(1024) void qcs_fft_tile(const QcPrep* __restrict__ pr, QcPrep pr1, int lo, int R, int lgC,
+                                                    int khi) {
+  __shared__ uint32_t L[1 << QCS_LB];
+  constexpr int TW = 1 << QCS_LB;
+  uint32_t* A = (pr ? pr[blockIdx.y] : pr1).ahat;
+  const int C = 1 << lgC;
+  const int64_t tile = blockIdx.x;
+  const int64_t cb = tile & ((((int64_t)1 << lo) >> lgC) - 1), hib = tile >> (lo - lgC);
+  auto gidx = [&](int t, int c) { return (hib << (lo + R)) | ((int64_t)t << lo) | (cb * C + c); };
+  for (int i = threadIdx.x; i < TW; i += blockDim.x) L[i] = A[gidx(i >> lgC, i & (C - 1))];
+  __syncthreads();
+  for (int k = khi; k >= lo; k--) {
+    const int s = k - lo;
+    for (int b = threadIdx.x; b < (TW >> 1); b += blockDim.x) {
+      const int c = b & (C - 1), bp = b >> lgC;
+      const int tu = ((bp >> s) << (s + 1)) | (bp & ((1 << s) - 1)), tv = tu | (1 << s);
+      const uint32_t cst = chunk_const(c_beta, (uint64_t)gidx(tu, c) >> (k + 1));
+      const int u = tu * C + c, v = tv * C + c;
+      const uint32_t xu = L[u] ^ gf32_mul_dev(cst, L[v]);
+      L[u] = xu;
+      L[v] ^= xu;
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < TW; i += blockDim.x) A[gidx(i >> lgC, i & (C - 1))] = L[i];
 }
 
 // ---- output: top inverse level + overlap-add + fold + x0 ---------------------
@@ -1188,6 +1269,25 @@ hipError_t prepare_batch(const QcPrep* pr, QcPrep pr1, int B, int m, hipStream_t
   }
   const dim3 blk(256);
   hipLaunchKernelGGL(qcs_pack, dim3(grid_for(M), (unsigned)B), blk, 0, st, pr, pr1, M);
+  if (m - 1 >= QCS_LB && m - QCS_LB <= QCS_LB) {
+    // upper conversion levels per launch, the rest in tiles; the FFT in two
+    // tiled launches (layers m-1..QCS_LB strided, QCS_LB-1..0 contiguous)
+    for (int k = m - 2; k >= QCS_LB; k--) {
+      hipLaunchKernelGGL(qcs_hi_fix, dim3(grid_for(Mh >> 1, 4096), (unsigned)B), blk, 0, st, pr, pr1, Mh, k);
+      hipLaunchKernelGGL(qcs_lo_update, dim3(grid_for(Mh >> 1, 4096), (unsigned)B), blk, 0, st, pr, pr1, Mh, k);
+    }
+    // 1024 lanes per 4096-word tile: the butterflies' scalar multiplies are
+    // serial chains, so the tiles need many waves to hide them
+    const dim3 big(1024);
+    hipLaunchKernelGGL(qcs_conv_tile, dim3((unsigned)(Mh >> QCS_LB), (unsigned)B), big, 0, st, pr, pr1,
+                       QCS_LB - 1);
+    const int Ru = m - QCS_LB;
+    hipLaunchKernelGGL(qcs_fft_tile, dim3((unsigned)(M >> QCS_LB), (unsigned)B), big, 0, st, pr, pr1, QCS_LB, Ru,
+                       QCS_LB - Ru, m - 1);
+    hipLaunchKernelGGL(qcs_fft_tile, dim3((unsigned)(M >> QCS_LB), (unsigned)B), big, 0, st, pr, pr1, 0, QCS_LB, 0,
+                       QCS_LB - 1);
+    return hipGetLastError();
+  }
   for (int k = m - 2; k >= 0; k--) {
     hipLaunchKernelGGL(qcs_hi_fix, dim3(grid_for(Mh >> 1, 4096), (unsigned)B), blk, 0, st, pr, pr1, Mh, k);
     hipLaunchKernelGGL(qcs_lo_update, dim3(grid_for(Mh >> 1, 4096), (unsigned)B), blk, 0, st, pr, pr1, Mh, k);

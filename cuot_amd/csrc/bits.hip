@@ -57,6 +57,49 @@ __global__ void sparse_encode_kernel(const uint32_t* __restrict__ a, int64_t n, 
   }
 }
 
+// n a multiple of 32 (and at least 64 words): the window of position s starts
+// at the same bit offset sb = s mod 32 in every word, so output word W takes
+// (a[W - so] << sb) | (a[W - so - 1] >> (32 - sb)) with so = s / 32 (indices mod
+// n / 32); SPR consecutive output words per lane share their loads.
+constexpr int SPR = 4;
+__global__ void sparse_encode_w32(const uint32_t* __restrict__ a, int64_t n, const int64_t* __restrict__ pos, int w,
+                                  uint32_t* __restrict__ out) {
+  const int64_t nw = n >> 5;
+  for (int64_t W0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * SPR; W0 < nw;
+       W0 += (int64_t)gridDim.x * blockDim.x * SPR) {
+    uint32_t acc[SPR];
+#pragma unroll
+    for (int r = 0; r < SPR; r++) acc[r] = 0u;
+    for (int q = 0; q < w; q++) {
+      const int64_t p = pos[q];
+      if (p < 0 || p >= 2 * n) continue;
+      if (p < n) {
+        const int64_t pw = (p >> 5) - W0;
+#pragma unroll
+        for (int r = 0; r < SPR; r++)
+          if (pw == r) acc[r] ^= 1u << (p & 31);
+        continue;
+      }
+      const int64_t sft = p - n;  // out_i ^= a_{(i - sft) mod n}
+      const int64_t so = sft >> 5;
+      const int sb = (int)(sft & 31);
+      int64_t i = W0 - so - 1;  // in [-nw, nw)
+      if (i < 0) i += nw;
+      uint32_t prev = a[i];
+#pragma unroll
+      for (int r = 0; r < SPR; r++) {
+        if (++i == nw) i = 0;
+        const uint32_t cur = a[i];
+        acc[r] ^= sb ? (cur << sb) | (prev >> (32 - sb)) : cur;
+        prev = cur;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < SPR; r++)
+      if (W0 + r < nw) out[W0 + r] = acc[r];
+  }
+}
+
 // ---- transpose ---------------------------------------------------------------
 CUOT_DEV uint32_t transpose32(uint32_t x, int lane) { return cuot_transpose32(x, lane); }
 
@@ -139,6 +182,11 @@ unsigned grid_for(int64_t total, int64_t cap = 1 << 16) {
 
 hipError_t cuot_launch_sparse(const uint32_t* a_words, int64_t n, const int64_t* pos, int w, uint32_t* out,
                               hipStream_t st) {
+  if ((n & 31) == 0 && (n >> 5) >= 64) {
+    hipLaunchKernelGGL(sparse_encode_w32, dim3(grid_for(((n >> 5) + SPR - 1) / SPR)), dim3(256), 0, st, a_words, n,
+                       pos, w, out);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(sparse_encode_kernel, dim3(grid_for((n + 31) / 32)), dim3(256), 0, st, a_words, n, pos, w,
                      out);
   return hipGetLastError();

@@ -22,6 +22,15 @@ namespace {
 
 constexpr int D = 10;
 
+// Waves per SIMD the LPN kernels are compiled for (a lower bound for the
+// register allocator; above 2 the rows of a unit are also kept apart).  The
+// gathers miss L2 and wait on the last-level cache, so residency matters:
+// computing each index AES block just before the first row that reads it
+// brought the kernel from 256 VGPRs (one wave per SIMD) to ~157 (three).
+#ifndef LPN_WAVES
+#define LPN_WAVES 1
+#endif
+
 struct LpnKArgs {
   const uint32_t* T0;
   uint32_t rk[44];
@@ -70,17 +79,19 @@ CUOT_DEV void lpn_unit(const AesLds& aes, const LpnKArgs& a, int64_t u) {
   if (lu < g4) {
     const int64_t i = start + 4 * lu;
     uint32_t w[4 * D];
-#pragma unroll
-    for (int m = 0; m < D; m++) {
+    // row q takes words 10q..10q+9, i.e. AES blocks 10q/4 .. (10q+9)/4: each
+    // block is computed just before the first row that reads it, and with
+    // LPN_WAVES > 2 the rows are kept apart, so that only a row's blocks and
+    // gathers are live at a time
+    auto block = [&](int m) {
       blk c = make_uint4((uint32_t)m, 0u, (uint32_t)i, (uint32_t)((uint64_t)i >> 32));
       blk e = aes_enc(aes, a.rk, c);
       w[4 * m + 0] = e.x;
       w[4 * m + 1] = e.y;
       w[4 * m + 2] = e.z;
       w[4 * m + 3] = e.w;
-    }
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
+    };
+    auto row = [&](int q) {
       blk acc[NR];
 #pragma unroll
       for (int rr = 0; rr < NR; rr++) acc[rr] = a.noise[rr][i + q];
@@ -92,7 +103,22 @@ CUOT_DEV void lpn_unit(const AesLds& aes, const LpnKArgs& a, int64_t u) {
       }
 #pragma unroll
       for (int rr = 0; rr < NR; rr++) a.out[rr][i + q] = acc[rr];
-    }
+      if (LPN_WAVES > 2) __builtin_amdgcn_sched_barrier(0);
+    };
+    block(0);
+    block(1);
+    block(2);
+    row(0);
+    block(3);
+    block(4);
+    row(1);
+    block(5);
+    block(6);
+    block(7);
+    row(2);
+    block(8);
+    block(9);
+    row(3);
   } else if (lu - g4 < t1) {
     const int64_t i = start + 4 * g4 + (lu - g4);
     uint32_t w[12];
@@ -119,7 +145,7 @@ CUOT_DEV void lpn_unit(const AesLds& aes, const LpnKArgs& a, int64_t u) {
 }
 
 template <int NR>
-__global__ __launch_bounds__(256) void lpn_kernel(LpnKArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LPN_WAVES))) void lpn_kernel(LpnKArgs a) {
   __shared__ uint32_t tab[CUOT_AES_LDS_WORDS];
   aes_stage_table(tab, a.T0, threadIdx.x, 256);
   __syncthreads();
@@ -131,7 +157,7 @@ __global__ __launch_bounds__(256) void lpn_kernel(LpnKArgs a) {
 // Batch: wg_start[i] = first WG of instance i (prefix of ceil(units / 256));
 // one WG covers 256 units of a single instance, so the round keys stay uniform.
 template <int NR>
-__global__ __launch_bounds__(256) void lpn_batch_kernel(const LpnKArgs* __restrict__ tabs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LPN_WAVES))) void lpn_batch_kernel(const LpnKArgs* __restrict__ tabs,
                                                         const int64_t* __restrict__ wg_start, int ninst) {
   __shared__ uint32_t tab[CUOT_AES_LDS_WORDS];
   const int64_t g = blockIdx.x;

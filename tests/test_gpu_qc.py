@@ -44,7 +44,9 @@ def test_qc_small_vs_brute(ctx, n):
     assert np.array_equal(qc_gpu(ctx, x, a, n), po.qc_encode_brute(x, a, n))
 
 
-@pytest.mark.parametrize("n", [1 << 14, 65537, 1 << 18])
+# 2^20 and 2^22 + 7 also cover the row conversion with several strings per
+# workgroup at 2^17 .. 2^19-bit strings (folded and general output)
+@pytest.mark.parametrize("n", [1 << 14, 65537, 1 << 18, 1 << 20, (1 << 22) + 7])
 def test_qc_vs_karatsuba(ctx, n):
     print(f"qc karatsuba n={n}", flush=True)
     r = rng(n + 1)
@@ -65,7 +67,8 @@ def test_qc_special_cases(ctx):
     assert np.array_equal(qc_gpu(ctx, x, xs, n), x[:n] ^ x[n + (np.arange(n) - s) % n])
 
 
-@pytest.mark.parametrize("n", [1, 33, 1000, 4096, 100003])
+# 4096 and 2^16 take the word-aligned path (n a multiple of 32, >= 64 words)
+@pytest.mark.parametrize("n", [1, 33, 1000, 4096, 100003, 1 << 16])
 def test_sparse_vs_oracle(ctx, n):
     import torch
 

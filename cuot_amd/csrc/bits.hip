@@ -130,9 +130,18 @@ __global__ __launch_bounds__(256) void transpose_fwd(const uint4* __restrict__ i
     }
     __syncthreads();
     const int64_t rw = N / 32;  // words per row
-    for (int f = threadIdx.x; f < 128 * 32; f += 256) {
-      const int k = f >> 5, tl = f & 31;
-      if (t0 + tl < ntile) rows[(int64_t)k * rw + t0 + tl] = sm[k][tl];
+    if ((N & 127) == 0) {
+      // rows of a multiple of 4 words: 16-byte stores (4 words of one row per lane)
+      for (int f = threadIdx.x; f < 128 * 8; f += 256) {
+        const int k = f >> 3, tl = (f & 7) * 4;
+        if (t0 + tl < ntile)
+          *(uint4*)&rows[(int64_t)k * rw + t0 + tl] = make_uint4(sm[k][tl], sm[k][tl + 1], sm[k][tl + 2], sm[k][tl + 3]);
+      }
+    } else {
+      for (int f = threadIdx.x; f < 128 * 32; f += 256) {
+        const int k = f >> 5, tl = f & 31;
+        if (t0 + tl < ntile) rows[(int64_t)k * rw + t0 + tl] = sm[k][tl];
+      }
     }
     __syncthreads();
   }
@@ -151,9 +160,22 @@ __global__ __launch_bounds__(256) void transpose_inv(const uint32_t* __restrict_
   const int64_t ntile = N / 32;
   const int64_t rw = N / 32;
   for (int64_t t0 = (int64_t)blockIdx.x * 32; t0 < ntile; t0 += (int64_t)gridDim.x * 32) {
-    for (int f = threadIdx.x; f < 128 * 32; f += 256) {
-      const int k = f >> 5, tl = f & 31;
-      sm[k][tl] = (t0 + tl < ntile) ? rows[(int64_t)k * rw + t0 + tl] : 0u;
+    if ((N & 127) == 0) {
+      // rows of a multiple of 4 words: 16-byte loads (4 words of one row per lane)
+      for (int f = threadIdx.x; f < 128 * 8; f += 256) {
+        const int k = f >> 3, tl = (f & 7) * 4;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (t0 + tl < ntile) v = *(const uint4*)&rows[(int64_t)k * rw + t0 + tl];
+        sm[k][tl] = v.x;
+        sm[k][tl + 1] = v.y;
+        sm[k][tl + 2] = v.z;
+        sm[k][tl + 3] = v.w;
+      }
+    } else {
+      for (int f = threadIdx.x; f < 128 * 32; f += 256) {
+        const int k = f >> 5, tl = f & 31;
+        sm[k][tl] = (t0 + tl < ntile) ? rows[(int64_t)k * rw + t0 + tl] : 0u;
+      }
     }
     __syncthreads();
 #pragma unroll

@@ -124,6 +124,22 @@ CUOT_DEV blk gf_mul(blk a, blk b) {
   return reduce256(p0, p1, p2, p3);
 }
 
+// bits of x at the even positions of a 64-bit word
+CUOT_DEV uint64_t spread32(uint32_t v) {
+  uint64_t x = v;
+  x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+  x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+  x = (x | (x << 2)) & 0x3333333333333333ull;
+  x = (x | (x << 1)) & 0x5555555555555555ull;
+  return x;
+}
+
+// a^2: in characteristic 2 the square of sum a_i x^i is sum a_i x^(2i)
+CUOT_DEV blk gf_sqr(blk a) {
+  return reduce256(spread32(a.x), spread32(a.y), spread32(a.z), spread32(a.w));
+}
+
 // a * x
 CUOT_DEV blk gf_mul_x(blk a) {
   const uint64_t l = lo64(a), h = hi64(a);
@@ -140,16 +156,6 @@ CUOT_DEV blk gf_mul_xj(blk a, int j) {
   const uint64_t w2 = s ? h >> (64 - s) : 0;
   if (j < 64) return reduce256(w0, w1, w2, 0);
   return reduce256(0, w0, w1, w2);
-}
-
-CUOT_DEV blk gf_pow(blk s, uint64_t e) {
-  blk r = make_uint4(1u, 0u, 0u, 0u);
-  while (e) {
-    if (e & 1) r = gf_mul(r, s);
-    s = gf_mul(s, s);
-    e >>= 1;
-  }
-  return r;
 }
 
 // c * K through the 4-bit window table T[k][v] = K * (v x^(4k))
@@ -179,26 +185,23 @@ __global__ __launch_bounds__(256) void consist_tree_kernel(ConsistArgs a) {
   __shared__ uint4 T[3][32][16];  // window tables of S = seed^P, seed, seed^16
   __shared__ uint4 red[2][256];
   __shared__ uint4 cs[4];
+  __shared__ uint4 PW[2][17];     // seed^b and seed^(16a), a, b = 0..16
   const int64_t tr = blockIdx.x;
   const int L = 1 << a.h;
   const int P = L < 256 ? L : 256;
   const int Mc = L / P;
   const int tid = threadIdx.x;
-  if (tid < 3) {
+  if (tid < 2) {
     const blk s = hash_seed(a.ssum[tr]);
     if (tid == 0) {  // S = s^P
       blk S = s;
-      for (int q = 1; q < P; q <<= 1) S = gf_mul(S, S);
+      for (int q = 1; q < P; q <<= 1) S = gf_sqr(S);
       cs[0] = S;
       cs[1] = s;
     } else if (tid == 1) {
       blk s16 = s;
-      for (int q = 0; q < 4; q++) s16 = gf_mul(s16, s16);
+      for (int q = 0; q < 4; q++) s16 = gf_sqr(s16);
       cs[2] = s16;
-    } else {
-      int64_t alpha = 0;
-      for (int d = 0; d < a.h; d++) alpha = (alpha << 1) + (a.bits[tr * a.h + d] ? 0 : 1);
-      cs[3] = (tr * L + alpha < a.n) ? gf_pow(s, (uint64_t)alpha + 1) : make_uint4(0u, 0u, 0u, 0u);
     }
   }
   __syncthreads();
@@ -219,6 +222,28 @@ __global__ __launch_bounds__(256) void consist_tree_kernel(ConsistArgs a) {
     T[q][k][v] = r;
   }
   __syncthreads();
+  // powers for the lanes' chi offsets: seed^(tid+1) = seed^(16a) * seed^b
+  // with a = (tid+1) >> 4 and b = (tid+1) & 15, one multiplication per lane
+  if (tid < 2) {
+    blk c = make_uint4(1u, 0u, 0u, 0u);
+    for (int i = 0; i < 17; i++) {
+      PW[tid][i] = c;
+      c = mul_tab(c, T[1 + tid]);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // chi_alpha = seed^(alpha+1) from the tables: alpha+1 = P*c + r, r < P
+    int64_t alpha = 0;
+    for (int d = 0; d < a.h; d++) alpha = (alpha << 1) + (a.bits[tr * a.h + d] ? 0 : 1);
+    blk v = make_uint4(0u, 0u, 0u, 0u);
+    if (tr * L + alpha < a.n) {
+      const int64_t e1 = alpha + 1, c = e1 / P, r = e1 % P;
+      v = gf_mul(PW[1][r >> 4], PW[0][r & 15]);
+      for (int64_t q = 0; q < c; q++) v = mul_tab(v, T[0]);
+    }
+    cs[3] = v;
+  }
   blk hs = make_uint4(0u, 0u, 0u, 0u), hr = hs;
   if (tid < P) {
     const int64_t base = tr * L + tid;
@@ -231,9 +256,7 @@ __global__ __launch_bounds__(256) void consist_tree_kernel(ConsistArgs a) {
         hr = bx(hr, a.lr[g]);
       }
     }
-    blk e = make_uint4(1u, 0u, 0u, 0u);  // seed^(tid+1)
-    for (int q = 0; q < ((tid + 1) >> 4); q++) e = mul_tab(e, T[2]);
-    for (int q = 0; q < ((tid + 1) & 15); q++) e = mul_tab(e, T[1]);
+    const blk e = gf_mul(PW[1][(tid + 1) >> 4], PW[0][(tid + 1) & 15]);  // seed^(tid+1)
     hs = gf_mul(e, hs);
     hr = gf_mul(e, hr);
   }

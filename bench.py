@@ -46,7 +46,51 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--qc-only", action="store_true",
+                    help="time only the quasi-cyclic encode of one party at n = 2^sot-n-log (after a 256 MiB "
+                         "device copy that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE) and print one JSON line")
     return ap.parse_args()
+
+
+def qc_only(args):
+    """The QC encoder alone: A^ of a random circulant, one untimed encode, then
+    `steps` encodes of random 2n-block input timed with HIP events (median).
+    A 256 MiB device-to-device copy first gives the hardware byte counters a
+    kernel of known traffic in the same process."""
+    import numpy as np
+    import torch
+
+    import cuot_amd
+
+    ctx = cuot_amd.Context(0)
+    n = 1 << args.sot_n_log
+    dev = "cuda"
+    g = torch.Generator().manual_seed(1)
+    a_bits = torch.randint(0, 256, ((n + 7) // 8 + 16,), dtype=torch.uint8, generator=g).to(dev)
+    ahat = torch.empty(ctx.qc_ahat_len(n), dtype=torch.int32, device=dev)
+    scratch = torch.empty(ctx.qc_scratch_bytes(n), dtype=torch.uint8, device=dev)
+    x = torch.randint(-2**62, 2**62, (2 * n, 2), dtype=torch.int64, generator=g).to(dev)
+    y = torch.empty((n, 2), dtype=torch.int64, device=dev)
+    ctx.qc_prepare(n, a_bits, ahat, scratch)
+    src = torch.empty(1 << 28, dtype=torch.uint8, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    for _ in range(max(1, args.warmup)):
+        ctx.qc_encode(n, ahat, x, y, scratch)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    for a, b in ev:
+        a.record()
+        ctx.qc_encode(n, ahat, x, y, scratch)
+        b.record()
+    torch.cuda.synchronize()
+    ms = float(np.median([a.elapsed_time(b) for a, b in ev]))
+    print(json.dumps({"workload": "qc encode, one party", "n": n, "steps": args.steps, "ms_per_encode": round(ms, 4),
+                      "compulsory_GBps": round(48 * n / (ms / 1e3) / 1e9, 1)}), flush=True)
+    ctx.close()
+    return 0
 
 
 def cpu_baseline(threads: int):
@@ -208,12 +252,14 @@ def serve_leg(torch, cuot_amd, ctx, count, rate, budget_gb):
         torch.cuda.synchronize()
         # batches are capped at 2^22 COTs so that a large request does not hold
         # small ones behind it for a whole 2^24-COT encode
-        lat, wall, nb = cuot_amd.serve_replay(ctx, insts, times, hbm_budget=budget, plain=plain,
-                                              max_batch=64, streams=2, max_batch_cots=1 << 22)
+        lat, wall, nb, fails = cuot_amd.serve_replay(ctx, insts, times, hbm_budget=budget, plain=plain,
+                                                     max_batch=64, streams=2, max_batch_cots=1 << 22,
+                                                     check_relation=True)
         res[name] = {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
                      "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
                      "mean_ms": round(float(lat.mean()) * 1e3, 3),
-                     "throughput_cots_s": round(total / wall, 1), "wall_s": round(wall, 3), "batches": nb}
+                     "throughput_cots_s": round(total / wall, 1), "wall_s": round(wall, 3), "batches": nb,
+                     "relation_ok": fails == 0, "relation_checked_results": count}
     return res
 
 
@@ -229,6 +275,8 @@ def main():
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is design-only)"}))
         return 2
+    if args.qc_only:
+        return qc_only(args)
     import torch
 
     import cuot_amd

@@ -255,10 +255,14 @@ class Server:
 
 
 def serve_replay(ctx: Context, instances, arrivals, hbm_budget: int, plain: bool = False, max_batch: int = 0,
-                 streams: int = 0, qc_chunk_bytes: int = 0, use_graph: bool = False, max_batch_cots: int = 0):
+                 streams: int = 0, qc_chunk_bytes: int = 0, use_graph: bool = False, max_batch_cots: int = 0,
+                 check_relation: bool = False):
     """Real-time replay of (arrival seconds, Instance) through a fresh server.
 
-    Returns (latencies in s as numpy array, wall seconds, batches launched)."""
+    Returns (latencies in s as numpy array, wall seconds, batches launched,
+    relation failures).  With check_relation every result is checked on the
+    device against its COT relation before release (failures = the number of
+    failing indices over all results); otherwise failures is None."""
     n = len(instances)
     cfg = ServerConfig(hbm_budget=hbm_budget, max_batch=max_batch, streams=streams, qc_chunk_bytes=qc_chunk_bytes,
                        use_graph=int(use_graph), plain=int(plain), max_batch_cots=max_batch_cots)
@@ -267,9 +271,10 @@ def serve_replay(ctx: Context, instances, arrivals, hbm_budget: int, plain: bool
     lat = (C.c_double * n)()
     wall = C.c_double()
     nb = C.c_int64()
-    check(ctx.lib.cuot_serve_replay(ctx.handle, C.byref(cfg), arr, t, n, lat, C.byref(wall), C.byref(nb)),
-          "cuot_serve_replay")
-    return np.array(lat[:], dtype=np.float64), wall.value, nb.value
+    fails = C.c_int64(-1)
+    check(ctx.lib.cuot_serve_replay(ctx.handle, C.byref(cfg), arr, t, n, lat, C.byref(wall), C.byref(nb),
+                                    C.byref(fails) if check_relation else None), "cuot_serve_replay")
+    return np.array(lat[:], dtype=np.float64), wall.value, nb.value, (fails.value if check_relation else None)
 
 
 def make_instance_struct(kind: int, n: int, t: int, h: int, k: int, seed, ntasks: int = 1, k0=(3242342, 0),

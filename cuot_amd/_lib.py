@@ -37,7 +37,7 @@ class SotConfig(C.Structure):
 
 
 class FerretConfig(C.Structure):
-    _fields_ = [("n", I64), ("k", I64), ("t", I64), ("h", I32), ("ntasks", I32), ("seed", Block)]
+    _fields_ = [("n", I64), ("k", I64), ("t", I64), ("h", I32), ("ntasks", I32), ("seed", Block), ("n_pre", I64)]
 
 
 class SilentConfig(C.Structure):
@@ -143,7 +143,7 @@ SIGNATURES = {
     "cuot_ferret_session_read_pre_files": (INT, [P, C.POINTER(FerretConfig), I64, C.c_char_p, C.c_char_p,
                                                  C.POINTER(P)]),
     "cuot_serve_replay": (INT, [P, C.POINTER(ServerConfig), C.POINTER(Instance), C.POINTER(C.c_double), INT,
-                                C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(I64)]),
+                                C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(I64), C.POINTER(I64)]),
 }
 
 _lib = None

@@ -51,17 +51,10 @@ CUOT_DEV uint32_t rotl32(uint32_t x, int s) { return (x << s) | (x >> (32 - s));
 // a ^ b ^ c as one three-input bit operation (table 0x96) on VGPR operands;
 // the round key (uniform in some kernels, per lane in others) is added by a
 // plain XOR, which takes it from either register file
-#ifndef AES_XOR3
-#define AES_XOR3 1
-#endif
 CUOT_DEV uint32_t aes_x3(uint32_t a, uint32_t b, uint32_t c) {
-#if AES_XOR3
   uint32_t r;
   asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "v"(c));
   return r;
-#else
-  return a ^ b ^ c;
-#endif
 }
 
 CUOT_DEV uint32_t aes_col(const AesLds& a, uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t k) {

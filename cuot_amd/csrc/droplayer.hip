@@ -63,7 +63,7 @@ extern "C" int cuot_mpcot_sender(cuot_ctx* ctx, cuot_block* const* expanded_d, c
   blk* ssum = (blk*)sep_d[0];
   blk* delta = ssum + T;
   // roots: the first t blocks of expanded (where the reference's level loop
-  // reads its level-0 input, dev_layer.cu:55-58); copied out before the
+  // reads its level-0 input, dev_layer.cu:35-38); copied out before the
   // leaves overwrite them
   CUOT_HIP_TRY(hipMemcpyAsync(roots, expanded_d[0], T * 16, hipMemcpyDeviceToDevice, st));
   CUOT_HIP_TRY(hipMemcpyAsync(delta, delta_h, 16, hipMemcpyHostToDevice, st));
@@ -256,7 +256,7 @@ extern "C" int cuot_sot_seed_expand(cuot_sot_role* r) {
 }
 
 // Receiver obtains the sender's masked level-sum rows (SOTRecver::get_punc_key,
-// silent/recver.cu:326-331) and forms its view: chosen sums and final row.
+// silent/recver.cu:117-122) and forms its view: chosen sums and final row.
 extern "C" int cuot_sot_get_punc_key(cuot_sot_role* recv, const cuot_sot_role* send) {
   CUOT_CHECK_ARG(recv && send, "role NULL");
   CUOT_CHECK_ARG(recv->role == CUOT_ROLE_RECVER && send->role == CUOT_ROLE_SENDER, "wrong roles");

@@ -163,6 +163,7 @@ int FerretPlan::validate() const {
   CUOT_CHECK_ARG(cfg.t >= 1 && cfg.t < (1ll << 31), "t out of range");
   CUOT_CHECK_ARG((cfg.t << cfg.h) >= cfg.n, "t * 2^h must cover n");
   CUOT_CHECK_ARG(cfg.k >= 1 && cfg.k < (1ll << 31), "k out of range");
+  CUOT_CHECK_ARG(cfg.k < cfg.n, "k must be below n (the LPN compresses k bootstrap COTs into n)");
   CUOT_CHECK_ARG(cfg.ntasks >= 1, "ntasks < 1");
   return CUOT_OK;
 }

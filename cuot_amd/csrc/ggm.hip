@@ -8,7 +8,7 @@
 //   A. top levels 0..top breadth-first in LDS (top = h for h <= 8, else
 //      top = h - L with L = min(h - 8, 6), so 2^top is 256..1024 nodes), level
 //      sums by a workgroup XOR reduction; the receiver resolves the sibling of
-//      its punctured path at every level (silent/recver.cu:307-324,
+//      its punctured path at every level (silent/recver.cu:98-115,
 //      spcot_recver.h:66-95).
 //   B. (h > 8) every lane expands 2^top/256 depth-`top` nodes depth-first for
 //      L levels, with a fully unrolled path stack in registers; leaves go
@@ -125,7 +125,7 @@ CUOT_DEV void bfs_levels(const AesLds& aes, const GgmKArgs& a, Lds& s, int64_t t
         if (final_rule && r == nlev - 1) {
           blk f = a.fin[tree];
           if (a.mode == CUOT_MODE_SOT) {
-            // silent/recver.cu:316-321 final layer: opposite-side sum
+            // silent/recver.cu:106-112 final layer: opposite-side sum
             s.nodes[np] = bxor(f, c ? L : R);
           } else {
             // spcot_recver.h:56-63: all other leaves, LSB cleared, xor secret

@@ -1,13 +1,14 @@
 // lpn.hip -- Ferret's 10-local primal LPN as one fused kernel:
 //   out[i] = noise[i] ^ XOR_{j<10} kk[idx(i, j)]
 // with the indices generated on the fly by AES under the LPN seed exactly as
-// emp-ot's LpnF2 does (ferret/emp-ot/emp-ot/ferret/lpn_f2.h:155-188):
+// emp-ot's LpnF2 does (ferret/emp-ot/emp-ot/ferret/lpn_f2.h:46-79):
 //   rows i..i+3 (a "__compute4" group): AES(makeBlock(i, m)) for m < 10 gives
 //     40 words; row i+q uses words 10q..10q+9, idx = w & mask, minus k if >= k;
 //   the last 1..4 rows of every task ("__compute1"): AES(makeBlock(i, m)) for
 //     m < 3, idx = w[j] % k.
 // Tasks split [0, n) in equal widths n/ntasks with the last task taking the
-// remainder (lpn_f2.h:209-221).  The reference GPU path materialises the index
+// remainder (LpnF2::bench, lpn_f2.h:100-115, commented out in the reference
+// tree; each task is LpnF2::task, lpn_f2.h:72-79).  The reference GPU path materialises the index
 // array and uses other counters (gpu/gpu_ops.cu:102-109, dev_layer.cu:146-186);
 // this kernel keeps the emp-ot definition and never writes indices to HBM.
 //
@@ -22,14 +23,10 @@ namespace {
 
 constexpr int D = 10;
 
-// Waves per SIMD the LPN kernels are compiled for (a lower bound for the
-// register allocator; above 2 the rows of a unit are also kept apart).  The
-// gathers miss L2 and wait on the last-level cache, so residency matters:
-// computing each index AES block just before the first row that reads it
-// brought the kernel from 256 VGPRs (one wave per SIMD) to ~157 (three).
-#ifndef LPN_WAVES
-#define LPN_WAVES 1
-#endif
+// The gathers miss L2 and wait on the last-level cache, so residency
+// matters: computing each index AES block just before the first row that
+// reads it brought the kernel from 256 VGPRs (one wave per SIMD) to ~157
+// (three).
 
 struct LpnKArgs {
   const uint32_t* T0;
@@ -80,9 +77,7 @@ CUOT_DEV void lpn_unit(const AesLds& aes, const LpnKArgs& a, int64_t u) {
     const int64_t i = start + 4 * lu;
     uint32_t w[4 * D];
     // row q takes words 10q..10q+9, i.e. AES blocks 10q/4 .. (10q+9)/4: each
-    // block is computed just before the first row that reads it, and with
-    // LPN_WAVES > 2 the rows are kept apart, so that only a row's blocks and
-    // gathers are live at a time
+    // block is computed just before the first row that reads it
     auto block = [&](int m) {
       blk c = make_uint4((uint32_t)m, 0u, (uint32_t)i, (uint32_t)((uint64_t)i >> 32));
       blk e = aes_enc(aes, a.rk, c);
@@ -103,7 +98,6 @@ CUOT_DEV void lpn_unit(const AesLds& aes, const LpnKArgs& a, int64_t u) {
       }
 #pragma unroll
       for (int rr = 0; rr < NR; rr++) a.out[rr][i + q] = acc[rr];
-      if (LPN_WAVES > 2) __builtin_amdgcn_sched_barrier(0);
     };
     block(0);
     block(1);
@@ -145,7 +139,7 @@ CUOT_DEV void lpn_unit(const AesLds& aes, const LpnKArgs& a, int64_t u) {
 }
 
 template <int NR>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LPN_WAVES))) void lpn_kernel(LpnKArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void lpn_kernel(LpnKArgs a) {
   __shared__ uint32_t tab[CUOT_AES_LDS_WORDS];
   aes_stage_table(tab, a.T0, threadIdx.x, 256);
   __syncthreads();
@@ -157,7 +151,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LPN_WAVES))
 // Batch: wg_start[i] = first WG of instance i (prefix of ceil(units / 256));
 // one WG covers 256 units of a single instance, so the round keys stay uniform.
 template <int NR>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LPN_WAVES))) void lpn_batch_kernel(const LpnKArgs* __restrict__ tabs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void lpn_batch_kernel(const LpnKArgs* __restrict__ tabs,
                                                         const int64_t* __restrict__ wg_start, int ninst) {
   __shared__ uint32_t tab[CUOT_AES_LDS_WORDS];
   const int64_t g = blockIdx.x;
@@ -193,7 +187,7 @@ bool fill_args(const LpnLaunch& l, LpnKArgs& a) {
   memcpy(a.rk, l.rk, sizeof(a.rk));
   a.n = l.n;
   a.k = l.k;
-  uint32_t mask = 1;  // lpn_f2.h:136-140
+  uint32_t mask = 1;  // lpn_f2.h:27-31
   while ((int64_t)mask < l.k) mask = (mask << 1) | 1u;
   a.mask = mask;
   a.ntasks = l.ntasks < 1 ? 1 : l.ntasks;

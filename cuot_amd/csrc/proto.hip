@@ -35,7 +35,7 @@ __global__ void setup_fix_kernel(blk* delta, const blk* alpha_blocks, int64_t t,
   }
 }
 
-// silent/sender.cu:170-192: M0[d] = m0b[d] ^ L_d, M1[d] = m1b[d] ^ R_d (d < h);
+// silent/sender.cu:80-103: M0[d] = m0b[d] ^ L_d, M1[d] = m1b[d] ^ R_d (d < h);
 // M0[h] = m0b[h-1] ^ R_{h-1} ^ Delta, M1[h] = m1b[h-1] ^ L_{h-1} ^ Delta.
 __global__ void sot_msgs_kernel(const blk* sums, const blk* m0b, const blk* m1b, const blk* delta, int64_t t,
                                 int h, blk* M0, blk* M1) {
@@ -53,7 +53,7 @@ __global__ void sot_msgs_kernel(const blk* sums, const blk* m0b, const blk* m1b,
   }
 }
 
-// The receiver's view (silent/recver.cu:352-365): mc = m_c base-OT message;
+// The receiver's view (silent/recver.cu:143-157): mc = m_c base-OT message;
 // chosen sum at level d = M_c[d] ^ mc[d]; final = M_c[h] ^ mc[h-1].
 __global__ void sot_recv_prep_kernel(const uint8_t* choice, const blk* M0, const blk* M1, const blk* m0b,
                                      const blk* m1b, int64_t t, int h, blk* csum, blk* fin) {
@@ -67,8 +67,8 @@ __global__ void sot_recv_prep_kernel(const uint8_t* choice, const blk* M0, const
   }
 }
 
-// Ferret: secret_sum = (L_h ^ R_h) & ~1 ^ Delta (spcot_sender.h:115-124) and
-// the receiver's chosen sums (OT of the level sums, mpcot_reg.h:352-363).
+// Ferret: secret_sum = (L_h ^ R_h) & ~1 ^ Delta (spcot_sender.h:49-58) and
+// the receiver's chosen sums (OT of the level sums, mpcot_reg.h:146-157).
 __global__ void ferret_sums_kernel(const blk* sums, const blk* delta, const uint8_t* choice, int64_t t, int h,
                                    blk* ssum, blk* csum, blk* lsum, blk* rsum) {
   const blk D = *delta;

@@ -15,6 +15,7 @@
 // at a time, each run by cuot_sot_run / cuot_ferret_run (FIFO plain loop).
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -98,6 +99,23 @@ size_t out_bytes_of(const cuot_instance& in) {
 }
 
 struct Flight;
+
+// COT relation of one served result: SOT w_i = v_i ^ b_i * Delta (b packed
+// LSB-first), Ferret r_i = s_i ^ lsb(r_i) * Delta with lsb(Delta) = 1.  Each
+// mismatching index adds one to *fail.
+__global__ void relation_check_kernel(const uint4* __restrict__ o0, const uint4* __restrict__ o1,
+                                      const uint32_t* __restrict__ bits, const uint4* __restrict__ delta, int64_t n,
+                                      int kind, unsigned long long* fail) {
+  const uint4 d = delta[0];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint4 a = o0[i], b = o1[i];
+    const bool sel = kind == CUOT_MODE_SOT ? ((bits[i >> 5] >> (i & 31)) & 1u) : (b.x & 1u);
+    const uint4 e = sel ? make_uint4(a.x ^ d.x, a.y ^ d.y, a.z ^ d.z, a.w ^ d.w) : a;
+    const bool bad = e.x != b.x || e.y != b.y || e.z != b.z || e.w != b.w || (kind != CUOT_MODE_SOT && i == 0 && !(d.x & 1u));
+    if (bad) atomicAdd(fail, 1ull);
+  }
+}
 
 }  // namespace
 
@@ -330,7 +348,10 @@ void cuot_server::loop() {
   hipSetDevice(ctx->device);
   std::unique_lock<std::mutex> lk(mu);
   for (;;) {
-    // reap finished batches
+    // reap finished batches; their plans (graphs, device tables) are freed
+    // after the lock is dropped, so that no HIP call runs while on_done may be
+    // waiting for the lock
+    std::vector<Flight*> finished;
     for (auto it = inflight.begin(); it != inflight.end();) {
       Flight* f = *it;
       if (!f->done.load()) {
@@ -338,19 +359,40 @@ void cuot_server::loop() {
         continue;
       }
       for (int64_t t : f->tickets) {
-        Req& r = reqs[t];
-        r.state = 2;
-        r.status = f->status;
-        r.t_done = f->t_done;
+        auto rit = reqs.find(t);
+        if (rit == reqs.end()) continue;
+        rit->second.state = 2;
+        rit->second.status = f->status;
+        rit->second.t_done = f->t_done;
       }
       arena.release(f->ws_off, f->ws_bytes);
       busy[f->stream] = false;
-      if (f->batch) cuot_batch_free(f->batch);
-      delete f;
+      finished.push_back(f);
       it = inflight.erase(it);
-      cv.notify_all();
     }
-    if (stop && inflight.empty()) break;
+    if (!finished.empty()) {
+      cv.notify_all();
+      lk.unlock();
+      for (Flight* f : finished) {
+        if (f->batch) cuot_batch_free(f->batch);
+        delete f;
+      }
+      lk.lock();
+    }
+    if (stop && inflight.empty()) {
+      // requests still queued at shutdown fail, and their waiters wake
+      const double td = now_s();
+      for (int64_t t : queue) {
+        auto rit = reqs.find(t);
+        if (rit == reqs.end()) continue;
+        rit->second.state = 2;
+        rit->second.status = CUOT_ERR_STATE;
+        rit->second.t_done = td;
+      }
+      queue.clear();
+      cv.notify_all();
+      break;
+    }
     bool launched = false;
     if (!stop) {
       for (size_t s = 0; s < streams.size() && !queue.empty(); s++) {
@@ -424,8 +466,16 @@ extern "C" int cuot_server_poll(cuot_server* s, int64_t ticket, int wait, cuot_s
   std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->reqs.find(ticket);
   CUOT_CHECK_ARG(it != s->reqs.end(), "unknown ticket");
-  if (wait) s->cv.wait(lk, [&] { return s->reqs[ticket].state == 2; });
-  const Req& r = s->reqs[ticket];
+  // a ticket released by another thread meanwhile counts as finished
+  if (wait) {
+    s->cv.wait(lk, [&] {
+      auto w = s->reqs.find(ticket);
+      return w == s->reqs.end() || w->second.state == 2;
+    });
+    it = s->reqs.find(ticket);
+    CUOT_CHECK_ARG(it != s->reqs.end(), "ticket released while waiting");
+  }
+  const Req& r = it->second;
   memset(res, 0, sizeof(*res));
   res->done = r.state == 2;
   res->status = r.status;
@@ -483,18 +533,74 @@ extern "C" int cuot_server_free(cuot_server* s) {
 // Replays an arrival schedule in real time (request i is submitted at
 // arrival_s[i] seconds after the start), waits for every request, releases
 // its outputs and reports its latency (completion - arrival) in latency_s.
+// With relation_failures != NULL every completed result is checked on the
+// device (relation_check_kernel, on a stream of its own, after its completion
+// time was taken) before its outputs are released; *relation_failures is the
+// number of indices, over all results, that break the COT relation.
 extern "C" int cuot_serve_replay(cuot_ctx* ctx, const cuot_server_config* cfg, const cuot_instance* reqs,
                                  const double* arrival_s, int count, double* latency_s, double* wall_s,
-                                 int64_t* batches) {
+                                 int64_t* batches, int64_t* relation_failures) {
   CUOT_CHECK_ARG(ctx && cfg && reqs && arrival_s && latency_s && count >= 1, "bad argument");
   cuot_server* s = nullptr;
   int rc = cuot_server_new(ctx, cfg, &s);
   if (rc) return rc;
+  const bool check = relation_failures != nullptr;
+  hipStream_t cst = nullptr;
+  unsigned long long* fail_d = nullptr;
+  if (check) {
+    if (hipStreamCreateWithFlags(&cst, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void**)&fail_d, sizeof(unsigned long long)) != hipSuccess ||
+        hipMemsetAsync(fail_d, 0, sizeof(unsigned long long), cst) != hipSuccess) {
+      if (fail_d) hipFree(fail_d);
+      if (cst) hipStreamDestroy(cst);
+      cuot_server_free(s);
+      cuot_set_last_error("relation check set-up failed", __FILE__, __LINE__);
+      return CUOT_ERR_HIP;
+    }
+  }
   std::vector<int64_t> tickets(count, 0);
   std::vector<double> arrive(count, 0);
   std::vector<int> pending;
+  std::vector<std::pair<int64_t, hipEvent_t>> checking;  // released once their check has run
   const double t0 = now_s();
   int first_err = CUOT_OK;
+  // record latency and status of a finished request, then check and release it
+  auto finish = [&](int j, const cuot_server_result& r) {
+    latency_s[j] = r.t_done - arrive[j];
+    if (r.status && !first_err) first_err = r.status;
+    if (check && r.status == CUOT_OK) {
+      const int64_t n = reqs[j].n;
+      const int blocks = (int)std::min<int64_t>(2048, (n + 255) / 256);
+      hipLaunchKernelGGL(relation_check_kernel, dim3(blocks), dim3(256), 0, cst, (const uint4*)r.out0,
+                         (const uint4*)r.out1, (const uint32_t*)r.bits, (const uint4*)r.delta, n, reqs[j].kind,
+                         fail_d);
+      hipEvent_t ev = nullptr;
+      if (hipGetLastError() != hipSuccess || hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+          hipEventRecord(ev, cst) != hipSuccess) {
+        if (!first_err) first_err = CUOT_ERR_HIP;
+        hipStreamSynchronize(cst);
+        if (ev) hipEventDestroy(ev);
+        cuot_server_release(s, tickets[j]);
+        return;
+      }
+      checking.emplace_back(tickets[j], ev);
+    } else {
+      cuot_server_release(s, tickets[j]);
+    }
+  };
+  auto release_checked = [&](bool all) {
+    for (size_t q = 0; q < checking.size();) {
+      if (all ? hipEventSynchronize(checking[q].second) == hipSuccess
+              : hipEventQuery(checking[q].second) == hipSuccess) {
+        hipEventDestroy(checking[q].second);
+        cuot_server_release(s, checking[q].first);
+        checking[q] = checking.back();
+        checking.pop_back();
+      } else {
+        q++;
+      }
+    }
+  };
   for (int i = 0; i < count; i++) {
     const double due = t0 + arrival_s[i];
     double tn = now_s();
@@ -512,9 +618,7 @@ extern "C" int cuot_serve_replay(cuot_ctx* ctx, const cuot_server_config* cfg, c
       const int j = pending[q];
       cuot_server_result r;
       if (cuot_server_poll(s, tickets[j], 0, &r) == CUOT_OK && r.done) {
-        latency_s[j] = r.t_done - arrive[j];
-        if (r.status && !first_err) first_err = r.status;
-        cuot_server_release(s, tickets[j]);
+        finish(j, r);
         tickets[j] = -tickets[j];
         pending[q] = pending.back();
         pending.pop_back();
@@ -522,16 +626,25 @@ extern "C" int cuot_serve_replay(cuot_ctx* ctx, const cuot_server_config* cfg, c
         q++;
       }
     }
+    release_checked(false);
   }
   for (int j = 0; j < count; j++) {
     if (tickets[j] <= 0) continue;
     cuot_server_result r;
     cuot_server_poll(s, tickets[j], 1, &r);
-    latency_s[j] = r.t_done - arrive[j];
-    if (r.status && !first_err) first_err = r.status;
-    cuot_server_release(s, tickets[j]);
+    finish(j, r);
+    release_checked(false);
   }
   if (wall_s) *wall_s = now_s() - t0;
+  release_checked(true);
+  if (check) {
+    unsigned long long fails = 0;
+    if (hipMemcpy(&fails, fail_d, sizeof(fails), hipMemcpyDeviceToHost) != hipSuccess && !first_err)
+      first_err = CUOT_ERR_HIP;
+    *relation_failures = (int64_t)fails;
+    hipFree(fail_d);
+    hipStreamDestroy(cst);
+  }
   if (batches) cuot_server_stats(s, batches, nullptr);
   cuot_server_free(s);
   return first_err;

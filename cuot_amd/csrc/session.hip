@@ -90,9 +90,10 @@ struct cuot_ferret_session {
   cuot_ctx* ctx = nullptr;
   FerretPlan plan;  // parameters, GGM/LPN argument builders
   int64_t npre = 0, M = 0, ot_limit = 0, round = 0;
+  int64_t n_pre_file = 0;  // blocks of ot_pre_data in the state blob / pre-OT files (>= M)
   bool seeded = false;  // the MPCOT half of the current extend has run
   bool malicious = false;  // run the consistency check after each MPCOT
-  bool checked = false;    // a consistency check has been enqueued
+  bool checked = false;    // a consistency check has been enqueued for the current extend
   blk* vwc = nullptr;      // per-tree V, W, chi_alpha (3 t blocks)
   uint8_t* chk = nullptr;  // x' (128), digests (2 x 32), ok (int32)
   char* mem = nullptr;
@@ -130,7 +131,7 @@ size_t carve_session(cuot_ferret_session* s, char* base) {
 }
 
 // the f2k consistency check on the current trees' leaves and the first 128
-// bootstrap COTs (mpcot_reg.h:94-95: pre_cot_data = ot_input)
+// bootstrap COTs (mpcot_reg.h:97-98: pre_cot_data = ot_input)
 hipError_t consistency(cuot_ferret_session* s, const blk* out_s, const blk* out_r, hipStream_t st) {
   const cuot_ferret_config& cfg = s->plan.cfg;
   s->checked = true;
@@ -142,6 +143,7 @@ hipError_t consistency(cuot_ferret_session* s, const blk* out_s, const blk* out_
 hipError_t extend_seed(cuot_ferret_session* s, blk* out_s, blk* out_r, hipStream_t st) {
   const cuot_ferret_config& cfg = s->plan.cfg;
   hipError_t e;
+  s->checked = false;  // a check result belongs to the extend whose MPCOT it checked
   // 1. pre-OT through the hash
   CcrhArgs c;
   memset(&c, 0, sizeof(c));
@@ -226,6 +228,13 @@ int session_alloc(cuot_ctx* ctx, const cuot_ferret_config* cfg, cuot_ferret_sess
   s->npre = (int64_t)cfg->h * cfg->t;
   s->M = cfg->k + s->npre + 128;
   s->ot_limit = cfg->n - s->M;
+  s->n_pre_file = cfg->n_pre ? cfg->n_pre : s->M;
+  if (s->n_pre_file < s->M) {
+    delete s;
+    cuot_set_last_error("n_pre must be 0 or at least k + h*t + 128 (LPN parameter not matched)", __FILE__,
+                        __LINE__);
+    return CUOT_ERR_ARG;
+  }
   if (s->ot_limit <= 0) {
     delete s;
     cuot_set_last_error("n must exceed k + h*t + 128 (the bootstrap carried to the next extend)", __FILE__,
@@ -252,10 +261,21 @@ void session_destroy(cuot_ferret_session* s) {
 // party ids as in emp-ot: ALICE = 1 (sender, holds Delta), BOB = 2 (receiver)
 constexpr int64_t kAlice = 1, kBob = 2;
 
-int64_t state_bytes(int64_t M) { return 4 * (int64_t)sizeof(int64_t) + 16 + 16 * M; }
+int64_t state_bytes(int64_t P) { return 4 * (int64_t)sizeof(int64_t) + 16 + 16 * P; }
 
-// copies one party's Delta (zero for the receiver) and bootstrap to the host
-// after all work on the device has finished
+// blocks of ot_pre_data in the blob and files: cfg->n_pre, or M when 0;
+// -1 when the parameters are invalid or n_pre < M
+int64_t pre_blocks(const cuot_ferret_config* cfg) {
+  if (!cfg || cfg->n <= 0 || cfg->k <= 0 || cfg->t <= 0 || cfg->h <= 0 || cfg->n_pre < 0) return -1;
+  const int64_t M = cfg->k + (int64_t)cfg->h * cfg->t + 128;
+  if (M >= cfg->n) return -1;
+  const int64_t P = cfg->n_pre ? cfg->n_pre : M;
+  return P >= M ? P : -1;
+}
+
+// copies one party's Delta (zero for the receiver) and its n_pre_file blocks
+// of ot_pre_data (the M bootstrap COTs, then zeros) to the host after all
+// work on the device has finished
 int export_party(const cuot_ferret_session* s, int64_t party, blk* delta_h, void* boot_h) {
   CUOT_HIP_TRY(hipSetDevice(s->ctx->device));
   CUOT_HIP_TRY(hipDeviceSynchronize());
@@ -265,6 +285,7 @@ int export_party(const cuot_ferret_session* s, int64_t party, blk* delta_h, void
     memset(delta_h, 0, 16);
   }
   CUOT_HIP_TRY(hipMemcpy(boot_h, party == kAlice ? s->boot_s : s->boot_r, (size_t)s->M * 16, hipMemcpyDeviceToHost));
+  memset((char*)boot_h + (size_t)s->M * 16, 0, (size_t)(s->n_pre_file - s->M) * 16);
   return CUOT_OK;
 }
 
@@ -315,15 +336,14 @@ extern "C" int cuot_ferret_session_new(cuot_ctx* ctx, const cuot_ferret_config* 
 }
 
 extern "C" int64_t cuot_ferret_state_size(const cuot_ferret_config* cfg) {
-  if (!cfg || cfg->n <= 0 || cfg->k <= 0 || cfg->t <= 0 || cfg->h <= 0) return -1;
-  const int64_t M = cfg->k + (int64_t)cfg->h * cfg->t + 128;
-  return M < cfg->n ? state_bytes(M) : -1;
+  const int64_t P = pre_blocks(cfg);
+  return P > 0 ? state_bytes(P) : -1;
 }
 
 extern "C" int cuot_ferret_session_assemble_state(const cuot_ferret_session* s, int party, void* data, int64_t size) {
   CUOT_CHECK_ARG(s && data, "NULL argument");
   CUOT_CHECK_ARG(party == kAlice || party == kBob, "party must be 1 (ALICE, sender) or 2 (BOB, receiver)");
-  CUOT_CHECK_ARG(size >= state_bytes(s->M), "state buffer smaller than cuot_ferret_state_size");
+  CUOT_CHECK_ARG(size >= state_bytes(s->n_pre_file), "state buffer smaller than cuot_ferret_state_size");
   unsigned char* a = (unsigned char*)data;
   const int64_t head[4] = {(int64_t)party, s->plan.cfg.n, (int64_t)s->plan.cfg.t, s->plan.cfg.k};
   memcpy(a, head, sizeof(head));
@@ -351,7 +371,7 @@ extern "C" int cuot_ferret_session_disassemble_state(cuot_ctx* ctx, const cuot_f
 extern "C" int cuot_ferret_session_write_pre_file(const cuot_ferret_session* s, int party, const char* path) {
   CUOT_CHECK_ARG(s && path, "NULL argument");
   CUOT_CHECK_ARG(party == kAlice || party == kBob, "party must be 1 (ALICE, sender) or 2 (BOB, receiver)");
-  std::vector<unsigned char> boot((size_t)s->M * 16);
+  std::vector<unsigned char> boot((size_t)s->n_pre_file * 16);
   blk delta;
   int rc = export_party(s, party, &delta, boot.data());
   if (rc) return rc;
@@ -375,7 +395,7 @@ extern "C" int cuot_ferret_session_read_pre_files(cuot_ctx* ctx, const cuot_ferr
   *out = nullptr;
   const int64_t need = cuot_ferret_state_size(cfg);
   CUOT_CHECK_ARG(need > 0, "invalid parameters");
-  const int64_t M = (need - 48) / 16;
+  const int64_t P = (need - 48) / 16;  // ot_pre_data blocks in the file; the first M are the bootstrap
   std::vector<unsigned char> boot[2];
   blk delta;
   const char* paths[2] = {alice_path, bob_path};
@@ -388,7 +408,7 @@ extern "C" int cuot_ferret_session_read_pre_files(cuot_ctx* ctx, const cuot_ferr
     if (party_ok && r == 0) ok = read_exact(f, &delta, 16);
     ok = ok && party_ok && read_exact(f, nkt, sizeof(nkt));
     const bool params_ok = ok && nkt[0] == cfg->n && nkt[1] == cfg->t && nkt[2] == cfg->k;
-    boot[r].resize((size_t)M * 16);
+    boot[r].resize((size_t)P * 16);
     ok = ok && params_ok && read_exact(f, boot[r].data(), boot[r].size());
     fclose(f);
     CUOT_CHECK_ARG(party_ok, "wrong party");
@@ -463,7 +483,7 @@ extern "C" int cuot_ferret_session_check(const cuot_ferret_session* s, int* ok, 
                                          uint8_t* dig_b) {
   CUOT_CHECK_ARG(s && ok, "NULL argument");
   if (!s->checked) {
-    cuot_set_last_error("no consistency check has run on this session", __FILE__, __LINE__);
+    cuot_set_last_error("no consistency check has run for the current extend", __FILE__, __LINE__);
     return CUOT_ERR_STATE;
   }
   uint8_t h[196];

@@ -29,16 +29,18 @@ ALICE, BOB = 1, 2  # emp-ot party ids
 
 
 class FerretCOT:
-    def __init__(self, ctx, n: int, k: int, t: int, h: int, seed, ntasks: int = 1, _handle=None):
+    def __init__(self, ctx, n: int, k: int, t: int, h: int, seed, ntasks: int = 1, n_pre: int = 0, _handle=None):
         import torch
 
         from . import to_block
 
         self.ctx = ctx
         self.lib = _lib.load()
-        cfg = FerretConfig(n=n, k=k, t=t, h=h, ntasks=ntasks, seed=to_block(seed))
+        # n_pre: blocks of emp-ot's ot_pre_data in the state blob and pre-OT
+        # files (PrimalLPNParameter::n_pre, constants.h:8-22); 0 = the bootstrap size M
+        cfg = FerretConfig(n=n, k=k, t=t, h=h, ntasks=ntasks, seed=to_block(seed), n_pre=n_pre)
         self._cfg = cfg
-        self._params = dict(n=n, k=k, t=t, h=h, seed=seed, ntasks=ntasks)
+        self._params = dict(n=n, k=k, t=t, h=h, seed=seed, ntasks=ntasks, n_pre=n_pre)
         if _handle is None:
             hnd = C.c_void_p()
             check(self.lib.cuot_ferret_session_new(ctx.handle, C.byref(cfg), C.byref(hnd)),
@@ -56,7 +58,7 @@ class FerretCOT:
         self.Delta = torch.empty((1, 2), dtype=torch.int64, device=dev)
         check(self.lib.cuot_ferret_session_delta(hnd, self.Delta.data_ptr(), self._stream()),
               "cuot_ferret_session_delta")
-        self.ot_used = self.ot_limit  # nothing generated yet (ferret_cot.hpp:77)
+        self.ot_used = self.ot_limit  # nothing generated yet (ferret_cot.hpp:76)
 
     def _stream(self):
         import torch
@@ -163,17 +165,17 @@ class FerretCOT:
 
     @classmethod
     def disassemble_state(cls, ctx, alice: bytes, bob: bytes, n: int, k: int, t: int, h: int, seed,
-                          round: int = 0, ntasks: int = 1):
+                          round: int = 0, ntasks: int = 1, n_pre: int = 0):
         """A session resumed from both parties' state blobs at `round`."""
         from . import to_block
 
-        cfg = FerretConfig(n=n, k=k, t=t, h=h, ntasks=ntasks, seed=to_block(seed))
+        cfg = FerretConfig(n=n, k=k, t=t, h=h, ntasks=ntasks, seed=to_block(seed), n_pre=n_pre)
         hnd = C.c_void_p()
         size = min(len(alice), len(bob))
         check(_lib.load().cuot_ferret_session_disassemble_state(ctx.handle, C.byref(cfg), round, alice, bob, size,
                                                                 C.byref(hnd)),
               "cuot_ferret_session_disassemble_state")
-        return cls(ctx, n, k, t, h, seed, ntasks, _handle=hnd)
+        return cls(ctx, n, k, t, h, seed, ntasks, n_pre, _handle=hnd)
 
     def write_pre_data128_to_file(self, party: int, filename: str):
         check(self.lib.cuot_ferret_session_write_pre_file(self.handle, party, filename.encode()),
@@ -181,16 +183,16 @@ class FerretCOT:
 
     @classmethod
     def read_pre_data128_from_file(cls, ctx, alice_file: str, bob_file: str, n: int, k: int, t: int, h: int, seed,
-                                   round: int = 0, ntasks: int = 1):
+                                   round: int = 0, ntasks: int = 1, n_pre: int = 0):
         """A session from both parties' pre-OT files (which are then removed)."""
         from . import to_block
 
-        cfg = FerretConfig(n=n, k=k, t=t, h=h, ntasks=ntasks, seed=to_block(seed))
+        cfg = FerretConfig(n=n, k=k, t=t, h=h, ntasks=ntasks, seed=to_block(seed), n_pre=n_pre)
         hnd = C.c_void_p()
         check(_lib.load().cuot_ferret_session_read_pre_files(ctx.handle, C.byref(cfg), round, alice_file.encode(),
                                                              bob_file.encode(), C.byref(hnd)),
               "cuot_ferret_session_read_pre_files")
-        return cls(ctx, n, k, t, h, seed, ntasks, _handle=hnd)
+        return cls(ctx, n, k, t, h, seed, ntasks, n_pre, _handle=hnd)
 
     def rcot(self, num: int):
         """num random COTs for both parties (sender blocks, receiver blocks)."""

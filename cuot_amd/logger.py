@@ -1,4 +1,4 @@
-"""Per-role event log in the reference's result-file format (gpu/logger.cu:20-97,
+"""Per-role event log in the reference's result-file format (gpu/logger.cu:20-95,
 gpu/logger.h:7-27), used by the SOT roles (silent.py) and the Ferret extend
 loop (ferret.py) the way the reference's drivers use it
 (ferret/emp-ot/test/ferret.cpp:61-66, ferret_cot.hpp:84-92).

@@ -66,12 +66,12 @@ int cuot_prg(cuot_ctx *ctx, const cuot_block *seed_h, uint64_t stream_id, uint64
  * AES_k1(s)^s.  Leaves are tree-major in natural order; global leaf index
  * g = tree*2^h + i is written only if g < n_out (truncation).  Replaces
  * Aes::expand + Mat::sum per level (gpu/aes_expand.cu:38-46,
- * gpu/gpu_matrix.cu:155-169) as driven by silent/sender.cu:166-198,
- * silent/recver.cu:333-372 and dev_layer.cu:40-164. */
+ * gpu/gpu_matrix.cu:155-169) as driven by silent/sender.cu:76-108,
+ * silent/recver.cu:124-163 and dev_layer.cu:40-164. */
 typedef struct {
   int64_t t;
   int32_t h;
-  int32_t mode; /* CUOT_MODE_FERRET clears leaf LSBs as spcot_sender.h:118-121 */
+  int32_t mode; /* CUOT_MODE_FERRET clears leaf LSBs as spcot_sender.h:52-55 */
   int64_t n_out;
   cuot_block k0, k1;
 } cuot_ggm_params;
@@ -87,8 +87,9 @@ int cuot_ggm_recver(cuot_ctx *ctx, const cuot_ggm_params *p, const uint8_t *choi
 
 /* ---- Ferret 10-local primal LPN -------------------------------------------
  * out[i] = noise[i] ^ XOR_{j<10} kk[idx(i,j)], indices from AES under `seed`
- * with emp-ot's counter and reduction rules (lpn_f2.h:155-188) over `ntasks`
- * equal tasks (lpn_f2.h:209-221; emp-ot's single-thread run is ntasks = 1).
+ * with emp-ot's counter and reduction rules (lpn_f2.h:46-79) over `ntasks`
+ * equal tasks (LpnF2::task per task, lpn_f2.h:72-79; the task split is the commented-out
+ * LpnF2::bench, lpn_f2.h:100-115; emp-ot's single-thread run is ntasks = 1).
  * out may alias noise.  Replaces cuda_primal_lpn's device work
  * (dev_layer.cu:146-186) without materialising the index array. */
 typedef struct {
@@ -188,6 +189,9 @@ typedef struct {
   int32_t h;
   int32_t ntasks;
   cuot_block seed;
+  int64_t n_pre; /* sessions only: blocks of emp-ot's ot_pre_data (PrimalLPNParameter::n_pre,
+                    constants.h:8-22) carried by the state blob and pre-OT files; 0 = M (see below);
+                    otherwise >= M */
 } cuot_ferret_config;
 
 enum {
@@ -215,7 +219,7 @@ int cuot_setdev(int gpu); /* cuda_setdev, dev_layer.h:9 */
 
 /* cuda_mpcot_sender, dev_layer.h:11-12.  expanded_d[0]: t*2^depth blocks;
  * on entry its first t blocks hold the tree roots (the reference's level loop
- * reads its level-0 input from there, dev_layer.cu:55-58), on return the
+ * reads its level-0 input from there, dev_layer.cu:35-38), on return the
  * leaves.  buffer_d/sep_d: caller scratch of t*2^depth blocks
  * (mpcot_reg.h:55-61).  lSum_h/rSum_h[tree*depth + level], secret_sum_h[tree]
  * are written on the host. */
@@ -223,7 +227,7 @@ int cuot_mpcot_sender(cuot_ctx *ctx, cuot_block *const *expanded_d, cuot_block *
                       cuot_block *const *sep_d, cuot_block *lSum_h, cuot_block *rSum_h, cuot_block *secret_sum_h,
                       const int *t, int depth, const cuot_block *delta_h, int ngpu);
 /* cuda_mpcot_recver, dev_layer.h:14-15.  cSum_h[tree*depth+level]: chosen
- * sums; choices_h[tree*depth+level]: bool bytes (mpcot_reg.h:352-363). */
+ * sums; choices_h[tree*depth+level]: bool bytes (mpcot_reg.h:146-157). */
 int cuot_mpcot_recver(cuot_ctx *ctx, cuot_block *const *expanded_d, cuot_block *const *buffer_d,
                       cuot_block *const *sep_d, const cuot_block *cSum_h, const cuot_block *secret_sum_h,
                       const int *t, int depth, const uint8_t *choices_h, int ngpu);
@@ -254,14 +258,14 @@ typedef struct {
 typedef struct cuot_sot_role cuot_sot_role;
 
 int cuot_sot_sender_new(cuot_ctx *ctx, const cuot_silent_config *cfg, const cuot_block *seed_h,
-                        cuot_sot_role **out); /* SOTSender::SOTSender, sender.cu:102-136 */
+                        cuot_sot_role **out); /* SOTSender::SOTSender, sender.cu:12-46 */
 int cuot_sot_recver_new(cuot_ctx *ctx, const cuot_silent_config *cfg, const cuot_block *seed_h,
-                        cuot_sot_role **out); /* SOTRecver::SOTRecver, recver.cu:220-256 */
+                        cuot_sot_role **out); /* SOTRecver::SOTRecver, recver.cu:11-47 */
 int cuot_sot_role_free(cuot_sot_role *r);
-int cuot_sot_base_ot(cuot_sot_role *r);       /* base_ot, sender.cu:152-164 / recver.cu:273-285 */
-int cuot_sot_seed_expand(cuot_sot_role *r);   /* seed_expand, sender.cu:166-198 / recver.cu:333-372 */
-int cuot_sot_get_punc_key(cuot_sot_role *recv, const cuot_sot_role *send); /* recver.cu:326-331 */
-int cuot_sot_dual_lpn(cuot_sot_role *r);      /* dual_lpn, sender.cu:200-209 / recver.cu:374-383 */
+int cuot_sot_base_ot(cuot_sot_role *r);       /* base_ot, sender.cu:62-74 / recver.cu:64-76 */
+int cuot_sot_seed_expand(cuot_sot_role *r);   /* seed_expand, sender.cu:76-108 / recver.cu:124-163 */
+int cuot_sot_get_punc_key(cuot_sot_role *recv, const cuot_sot_role *send); /* recver.cu:117-122 */
+int cuot_sot_dual_lpn(cuot_sot_role *r);      /* dual_lpn, sender.cu:110-119 / recver.cu:165-174 */
 /* fullVector / puncVector (n blocks), choiceVector (receiver), delta (sender) */
 int cuot_sot_outputs(const cuot_sot_role *r, cuot_block **vec_d, uint8_t **choice_bits_d, cuot_block **delta_d,
                      int64_t *n);
@@ -346,15 +350,22 @@ typedef struct {
 typedef struct cuot_server cuot_server;
 int cuot_server_new(cuot_ctx *ctx, const cuot_server_config *cfg, cuot_server **out);
 int cuot_server_submit(cuot_server *s, const cuot_instance *inst, int64_t *ticket);
-/* wait != 0 blocks until the request is done */
+/* wait != 0 blocks until the request is done.  cuot_server_free fails the
+ * requests still queued (status CUOT_ERR_STATE) and wakes their waiters; it
+ * must not be called while another thread is inside a cuot_server_* call on
+ * the same server. */
 int cuot_server_poll(cuot_server *s, int64_t ticket, int wait, cuot_server_result *res);
 int cuot_server_release(cuot_server *s, int64_t ticket);
 int cuot_server_stats(cuot_server *s, int64_t *batches, size_t *arena_used);
 int cuot_server_free(cuot_server *s);
 /* Real-time replay of an arrival schedule through a fresh server; latency_s[i]
- * = completion - scheduled arrival of request i. */
+ * = completion - scheduled arrival of request i.  relation_failures != NULL:
+ * every result is checked on the device against its COT relation (SOT
+ * w = v ^ b*Delta, Ferret r = s ^ lsb(r)*Delta) before its release, and the
+ * number of failing indices over all results is returned there. */
 int cuot_serve_replay(cuot_ctx *ctx, const cuot_server_config *cfg, const cuot_instance *reqs,
-                      const double *arrival_s, int count, double *latency_s, double *wall_s, int64_t *batches);
+                      const double *arrival_s, int count, double *latency_s, double *wall_s, int64_t *batches,
+                      int64_t *relation_failures);
 
 /* ---- autotuner cost model ---------------------------------------------------------
  * No counterpart in the reference.  A small network (F inputs -> H tanh ->
@@ -423,7 +434,7 @@ int cuot_ferret_session_extend(cuot_ferret_session *s, cuot_block *out_s_d, cuot
 int cuot_ferret_session_extend_stages(cuot_ferret_session *s, cuot_block *out_s_d, cuot_block *out_r_d, int stages,
                                       void *stream);
 int cuot_ferret_session_free(cuot_ferret_session *s);
-/* Malicious mode (MpcotReg::set_malicious, mpcot_reg.h:68-70): every MPCOT is
+/* Malicious mode (MpcotReg::set_malicious, mpcot_reg.h:69-71): every MPCOT is
  * followed by the f2k consistency check of both parties (spcot_sender.h:88-98,
  * spcot_recver.h:97-107, mpcot_reg.h:177-220) with emp-tool's arithmetic
  * (SHA-256 seeds, chi_j = seed^(j+1) and inner products in GF(2^128) mod
@@ -431,9 +442,10 @@ int cuot_ferret_session_free(cuot_ferret_session *s);
  * Leaves not stored (global index >= n) are left out, and so is the
  * punctured leaf's chi when that leaf is not stored. */
 int cuot_ferret_session_set_malicious(cuot_ferret_session *s, int on);
-/* the last check (waits for the device): ok = 1 when ALICE's and BOB's
- * digests agree; x' (128 bytes, BOB's message), digests (32 bytes each);
- * CUOT_ERR_STATE before any check has run */
+/* the check of the current extend (waits for the device): ok = 1 when
+ * ALICE's and BOB's digests agree; x' (128 bytes, BOB's message), digests (32
+ * bytes each); CUOT_ERR_STATE when no check ran for the latest extend (the
+ * MPCOT stage of an extend discards the previous extend's result) */
 int cuot_ferret_session_check(const cuot_ferret_session *s, int *ok, uint8_t *xprime, uint8_t *dig_a,
                               uint8_t *dig_b);
 
@@ -465,16 +477,18 @@ int cuot_cot_to_rot(cuot_ctx *ctx, const cuot_block *data_s_d, const cuot_block 
 
 /* ---- session state blob and pre-OT files (emp-ot byte layouts) --------------------
  * party: 1 = ALICE (sender, holds Delta), 2 = BOB (receiver).  The bootstrap of
- * M = k + h*t + 128 COTs plays emp-ot's ot_pre_data of n_pre blocks.
+ * M = k + h*t + 128 COTs is the head of emp-ot's ot_pre_data of P = cfg->n_pre
+ * blocks (P = M when cfg->n_pre is 0); blocks [M, P) are written as zeros and
+ * ignored on reading (emp-ot's extend reads only the first M).
  * State blob (FerretCOT::assemble_state / disassemble_state / state_size,
  * ferret_cot.hpp:309-347): int64 party, n, t, k; 16-byte Delta (zero for BOB);
- * M blocks.  The session stays usable after assembling.  Loading checks the
- * party and (n, t, k) like disassemble_state (mismatch -> CUOT_ERR_ARG) and
- * resumes at `round` (the roots of round r come from the seed's stream 1 at
- * r*t, a counter the emp-ot blob does not carry).
+ * P blocks, 48 + 16 P bytes.  The session stays usable after assembling.
+ * Loading checks the party and (n, t, k) like disassemble_state (mismatch ->
+ * CUOT_ERR_ARG) and resumes at `round` (the roots of round r come from the
+ * seed's stream 1 at r*t, a counter the emp-ot blob does not carry).
  * Pre-OT file (write_pre_data128_to_file / read_pre_data128_from_file,
  * ferret_cot.hpp:215-247): int64 party; ALICE only: 16-byte Delta; int64 n,
- * t, k; M blocks.  Reading checks party and parameters ("wrong party",
+ * t, k; P blocks.  Reading checks party and parameters ("wrong party",
  * "wrong parameters") and removes both files afterwards, as emp-ot does. */
 int64_t cuot_ferret_state_size(const cuot_ferret_config *cfg); /* -1 if invalid */
 int cuot_ferret_session_assemble_state(const cuot_ferret_session *s, int party, void *data, int64_t size);

@@ -144,7 +144,7 @@ void or_aes_ecb(const uint8_t rk[176], const orb *in, orb *out, size_t n, int im
 /* ======================================================================== */
 /* Seed derivation: AES-CTR keyed by the 128-bit seed, counter block        */
 /* {lo = index, hi = stream}.  (Workload definition; the reference draws    */
-/* from rand(), silent/sender.cu:107-119, which is not reproducible.)       */
+/* from rand(), silent/sender.cu:110-119, which is not reproducible.)       */
 /* ======================================================================== */
 void or_prg(const orb *seed, uint64_t stream, uint64_t start, size_t count, orb *out) {
   uint8_t rk[176];
@@ -168,7 +168,7 @@ static inline orb bx(orb a, orb b) {
 }
 
 /* expand level in place: nodes[0..w) -> nodes[0..2w), iterating from the
- * top as node_expand_4to8 does (spcot_sender.h:139-149). */
+ * top as node_expand_4to8 does (spcot_sender.h:70-84). */
 static void ggm_level_inplace(const uint8_t rk0[176], const uint8_t rk1[176], orb *nodes, int64_t w) {
   orb p[8], l[8], r[8];
   int64_t i = w;
@@ -209,13 +209,13 @@ static const orb ORB_ZERO = {0, 0};
 static const orb ORB_LSB_CLEAR = {0xFFFFFFFFFFFFFFFEull, 0xFFFFFFFFFFFFFFFFull};
 
 static void ferret_keys(uint8_t rk0[176], uint8_t rk1[176]) {
-  /* TwoKeyPRP(zero_block, makeBlock(0, 1))  (spcot_sender.h:129) */
+  /* TwoKeyPRP(zero_block, makeBlock(0, 1))  (spcot_sender.h:63) */
   orb k0 = {0, 0}, k1 = {1, 0};
   or_aes_expand(&k0, rk0);
   or_aes_expand(&k1, rk1);
 }
 
-/* spcot_sender.h:115-152 (ggm_tree_gen) per tree; mpcot_reg.h:336-350
+/* spcot_sender.h:49-86 (ggm_tree_gen) per tree; mpcot_reg.h:130-144
  * concatenates trees; leaves beyond n_out are dropped (truncation). */
 void or_ferret_mpcot_sender(const orb *roots, int64_t t, int h, orb delta, int64_t n_out, orb *leaves,
                             orb *lsum, orb *rsum, orb *secret_sum, int nthreads) {
@@ -289,7 +289,7 @@ void or_ferret_mpcot_recver(int64_t t, int h, const uint8_t *b, const orb *csum,
   }
 }
 
-/* silent/sender.cu:166-198: masks base-OT rows with the level sums; the
+/* silent/sender.cu:76-108: masks base-OT rows with the level sums; the
  * extra row h is base row h-1 xor the opposite sum xor Delta. */
 void or_sot_sender_expand(const uint8_t rk0[176], const uint8_t rk1[176], const orb *roots, int64_t t,
                           int h, orb delta, const orb *m0b, const orb *m1b, int64_t n_leaves, orb *leaves,
@@ -320,7 +320,7 @@ void or_sot_sender_expand(const uint8_t rk0[176], const uint8_t rk1[176], const 
   }
 }
 
-/* silent/recver.cu:307-372 (fill_tree + seed_expand), with the receiver's
+/* silent/recver.cu:98-163 (fill_tree + seed_expand), with the receiver's
  * own unknown root taken as zero: the garbage on the punctured path cancels
  * in the fill rule val ^= fullSum ^ ownSideSum. */
 void or_sot_recver_expand(const uint8_t rk0[176], const uint8_t rk1[176], int64_t t, int h,
@@ -573,10 +573,10 @@ void or_bit_untranspose(const uint8_t *rows, int64_t N, orb *out) {
 }
 
 /* ======================================================================== */
-/* Ferret primal LPN, lpn_f2.h:155-188 (__compute4, __compute1, task).       */
+/* Ferret primal LPN, lpn_f2.h:46-79 (__compute4, __compute1, task).       */
 /* ======================================================================== */
 static int64_t lpn_mask(int64_t k) {
-  int64_t mask = 1; /* lpn_f2.h:136-140 */
+  int64_t mask = 1; /* lpn_f2.h:27-31 */
   while (mask < k) {
     mask <<= 1;
     mask |= 1;

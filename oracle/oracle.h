@@ -42,7 +42,7 @@ extern "C" {
 
 /* 128-bit block in memory byte order: bytes 0..7 = lo (little endian),
  * bytes 8..15 = hi.  emp-tool makeBlock(hi, lo) == {lo, hi}
- * (spcot_sender.h:118 clears bit 0 of byte 0 through makeBlock(~0, ~0-1)). */
+ * (spcot_sender.h:52 clears bit 0 of byte 0 through makeBlock(~0, ~0-1)). */
 typedef struct {
   uint64_t lo, hi;
 } orb;
@@ -64,14 +64,14 @@ void or_ggm_sender_tree(const uint8_t rk0[176], const uint8_t rk1[176], orb root
                         int h, orb *leaves, orb *lsum, orb *rsum);
 
 /* Ferret MPCOT, emp-ot semantics, t trees of depth h, leaves truncated to
- * n_out (<= t*2^h).  Sender: spcot_sender.h:115-152; receiver:
+ * n_out (<= t*2^h).  Sender: spcot_sender.h:49-86; receiver:
  * spcot_recver.h:53-95.  b[t*h + d] = receiver choice bit at level d. */
 void or_ferret_mpcot_sender(const orb *roots, int64_t t, int h, orb delta, int64_t n_out,
                             orb *leaves, orb *lsum, orb *rsum, orb *secret_sum, int nthreads);
 void or_ferret_mpcot_recver(int64_t t, int h, const uint8_t *b, const orb *csum,
                             const orb *secret_sum, int64_t n_out, orb *leaves, int nthreads);
 
-/* SOT seed expansion (silent/sender.cu:166-198, silent/recver.cu:307-372)
+/* SOT seed expansion (silent/sender.cu:76-108, silent/recver.cu:98-163)
  * with the config's child = AES_k(s) xor s.  m0b/m1b: base-OT messages
  * [h][t]; c[d*t + tr] receiver choice bits; M0/M1 out: [h+1][t]. */
 void or_sot_sender_expand(const uint8_t rk0[176], const uint8_t rk1[176], const orb *roots,
@@ -100,9 +100,10 @@ void or_bpmul(const uint64_t *a, const uint64_t *b, int64_t m, uint64_t *out);
 void or_bit_transpose(const orb *in, int64_t N, uint8_t *rows);
 void or_bit_untranspose(const uint8_t *rows, int64_t N, orb *out);
 
-/* ---- Ferret 10-local primal LPN (lpn_f2.h:155-188) --------------------- */
+/* ---- Ferret 10-local primal LPN (lpn_f2.h:46-79) --------------------- */
 /* nn[i] ^= XOR_j kk[idx(i,j)] for i in [0, n); PRP keyed with seed;
- * ntasks equal-width tasks as in LpnF2::bench (lpn_f2.h:209-221). */
+ * ntasks equal-width tasks as in LpnF2::bench (lpn_f2.h:100-115, commented out
+ * in the reference tree; each task runs LpnF2::task, lpn_f2.h:72-79). */
 void or_ferret_lpn(orb *nn, int64_t n, const orb *kk, int64_t k, orb seed, int d, int ntasks,
                    int nthreads);
 /* indices only (for tests): idx[i*d + j] */
@@ -120,7 +121,7 @@ void or_ferret_lpn_indices(int64_t n, int64_t k, orb seed, int d, int ntasks, ui
  *   8: Ferret bootstrap choice c_j = bit j%128 of block j/128;
  *      receiver's M_j = K_j ^ c_j * Delta (LSB = c_j, base_cot.h:59-70)
  * The receiver's choice at level d is c = 1 - bit (h-1-d) of alpha
- * (silent/recver.cu:288-296, spcot_recver.h:33-41). */
+ * (silent/recver.cu:78-87, spcot_recver.h:33-41). */
 typedef struct {
   int64_t n; /* COTs out; leaves = 2n = t * 2^h (last tree truncated) */
   int64_t t;

@@ -66,4 +66,12 @@ def test_corrupted_leaf_fails(ctx):
     ref = po.FerretSession(n, k, t, h, np.array(SEED, dtype=np.uint64), nthreads=8)
     s, r = ref.extend()
     assert np.array_equal(to_np(fc.out_s), s) and np.array_equal(to_np(fc.out_r), r)
+    # the next extend's MPCOT without a check discards the last result: the
+    # previous extend's digests are not reported as the current one's
+    assert lib.cuot_ferret_session_extend_stages(*args, 1, st) == 0
+    with pytest.raises(CuotError):
+        fc.check_result()
+    assert lib.cuot_ferret_session_extend_stages(*args, 2, st) == 0
+    with pytest.raises(CuotError):
+        fc.check_result()
     fc.close()

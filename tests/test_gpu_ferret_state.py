@@ -138,3 +138,46 @@ def test_staged_extend_and_log(ctx, tmp_path):
             [["m", str(e)] for e in range(5)]
         assert float(lines[14 + 2].split()[2]) > 0 and float(lines[14 + 3].split()[2]) > 0
     fc.close()
+
+
+def test_state_and_files_with_n_pre_above_bootstrap(ctx, tmp_path):
+    """emp-ot's blob and files carry param.n_pre blocks of ot_pre_data
+    (ferret_cot.hpp:226,238,318,346), more than the M the extend reads: with
+    n_pre > M the M bootstrap COTs are followed by zeros up to n_pre, the sizes
+    are emp-ot's state_size (48 + 16 n_pre) and file length, and sessions read
+    back from both continue byte-identically to the oracle."""
+    import torch
+
+    from cuot_amd import CuotError
+    from cuot_amd.ferret import ALICE, BOB, FerretCOT
+
+    print("ferret state with n_pre > M", flush=True)
+    fc, ref = _pair(ctx, 1)
+    M = ref.M
+    n_pre = M + 77
+    fcp = FerretCOT.disassemble_state(ctx, fc.assemble_state(ALICE), fc.assemble_state(BOB), seed=SEED, round=1,
+                                      n_pre=n_pre, **P)
+    a, b = fcp.assemble_state(ALICE), fcp.assemble_state(BOB)
+    assert len(a) == fcp.state_size() == 48 + 16 * n_pre
+    bs, br = ref.bootstrap()
+    pad = np.zeros((n_pre - M, 2), np.uint64)
+    assert a == po.ferret_state_blob(1, P["n"], P["t"], P["k"], ref.delta, np.concatenate([bs, pad]))
+    assert b == po.ferret_state_blob(2, P["n"], P["t"], P["k"], np.zeros(2, np.uint64), np.concatenate([br, pad]))
+    fa, fb = str(tmp_path / "send"), str(tmp_path / "recv")
+    fcp.write_pre_data128_to_file(ALICE, fa)
+    fcp.write_pre_data128_to_file(BOB, fb)
+    assert open(fa, "rb").read() == po.ferret_pre_file(1, P["n"], P["t"], P["k"], ref.delta, np.concatenate([bs, pad]))
+    assert os.path.getsize(fb) == 8 + 24 + 16 * n_pre
+    fcf = FerretCOT.read_pre_data128_from_file(ctx, fa, fb, seed=SEED, round=1, n_pre=n_pre, **P)
+    fcs = FerretCOT.disassemble_state(ctx, a, b, seed=SEED, round=1, n_pre=n_pre, **P)
+    fcf.extend_f2k()
+    fcs.extend_f2k()
+    torch.cuda.synchronize()
+    s, r = ref.extend()
+    for x in (fcf, fcs):
+        assert np.array_equal(to_np(x.out_s), s) and np.array_equal(to_np(x.out_r), r)
+    # n_pre below M is emp-ot's "LPN parameter not matched"
+    with pytest.raises(CuotError):
+        FerretCOT(ctx, seed=SEED, n_pre=M - 1, **P)
+    for x in (fc, fcp, fcf, fcs):
+        x.close()

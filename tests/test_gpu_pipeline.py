@@ -54,8 +54,9 @@ def relation_holds_gpu(v, w, bits_u8, d, n):
     return bool(torch.equal(expect, w)), int(bits.sum().item())
 
 
+# the last case is the CPU-path workload: n = 2^20 COTs, t = 256 trees of depth 13
 @pytest.mark.parametrize("n,t,h", [(1 << 10, 16, 7), (1000, 16, 7), (1 << 14, 32, 10), (1 << 16, 64, 11),
-                                   (3 << 14, 96, 10)])
+                                   (3 << 14, 96, 10), (1 << 20, 256, 13)])
 def test_sot_matches_oracle(ctx, n, t, h):
     print(f"sot oracle parity n={n} t={t} h={h}", flush=True)
     v, w, b, d = run_sot_gpu(ctx, n, t, h, SEED)

@@ -89,8 +89,10 @@ def test_replay_completes(ctx):
     reqs = _requests(40, lo_log=10, hi_log=15, salt=11)
     r = np.random.default_rng(3)
     arrivals = np.cumsum(r.exponential(1 / 400.0, size=len(reqs)))
-    lat, wall, nb = cuot_amd.serve_replay(ctx, [ins for _, ins in reqs], arrivals, hbm_budget=1 << 30, max_batch=8)
+    lat, wall, nb, fails = cuot_amd.serve_replay(ctx, [ins for _, ins in reqs], arrivals, hbm_budget=1 << 30,
+                                                 max_batch=8, check_relation=True)
     print(f"replay: wall {wall:.3f} s, batches {nb}, p50 {np.percentile(lat, 50) * 1e3:.2f} ms, "
           f"p99 {np.percentile(lat, 99) * 1e3:.2f} ms", flush=True)
     assert np.all(np.isfinite(lat)) and np.all(lat > 0)
     assert wall >= arrivals[-1]
+    assert fails == 0, f"{fails} indices of the served results break the COT relation"

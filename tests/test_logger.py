@@ -1,5 +1,5 @@
 """The event log (cuot_amd.logger) against the reference's result-file format
-(gpu/logger.cu:20-97) on a scripted clock and memory counter: header, the
+(gpu/logger.cu:20-95) on a scripted clock and memory counter: header, the
 ignore_init rule (CudaInit untimed, time origin at the first other start),
 float32 millisecond arithmetic printed with 6 significant digits, per-sample
 durations and memory growth at close."""

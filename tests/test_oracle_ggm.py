@@ -44,7 +44,7 @@ def test_tree_matches_recursive_definition(h):
 
 
 def _alpha_to_choice(alpha, h):
-    # alpha bit (h-1-d) = 1 - c[d]  (spcot_recver.h:33-41, silent/recver.cu:288-296)
+    # alpha bit (h-1-d) = 1 - c[d]  (spcot_recver.h:33-41, silent/recver.cu:78-87)
     return np.array([1 - ((alpha >> (h - 1 - d)) & 1) for d in range(h)], dtype=np.uint8)
 
 

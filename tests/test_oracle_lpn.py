@@ -1,4 +1,4 @@
-"""Pin the oracle's Ferret 10-local LPN (lpn_f2.h:155-188).
+"""Pin the oracle's Ferret 10-local LPN (lpn_f2.h:46-79).
 
 emp-tool is not available, so the index rule is checked against a second,
 independent restatement written from lpn_f2.h in Python (AES from the

@@ -155,8 +155,14 @@ def test_state_and_files_with_n_pre_above_bootstrap(ctx, tmp_path):
     fc, ref = _pair(ctx, 1)
     M = ref.M
     n_pre = M + 77
-    fcp = FerretCOT.disassemble_state(ctx, fc.assemble_state(ALICE), fc.assemble_state(BOB), seed=SEED, round=1,
-                                      n_pre=n_pre, **P)
+    # a blob of the n_pre = M session is too short for a session with more pre-OT blocks
+    with pytest.raises(CuotError):
+        FerretCOT.disassemble_state(ctx, fc.assemble_state(ALICE), fc.assemble_state(BOB), seed=SEED, round=1,
+                                    n_pre=n_pre, **P)
+    fcp = FerretCOT(ctx, seed=SEED, n_pre=n_pre, **P)
+    fcp.extend_f2k()
+    torch.cuda.synchronize()
+    assert np.array_equal(to_np(fcp.out_s), to_np(fc.out_s)) and np.array_equal(to_np(fcp.out_r), to_np(fc.out_r))
     a, b = fcp.assemble_state(ALICE), fcp.assemble_state(BOB)
     assert len(a) == fcp.state_size() == 48 + 16 * n_pre
     bs, br = ref.bootstrap()

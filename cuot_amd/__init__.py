@@ -138,6 +138,12 @@ class Context:
                                        _ptr(out_s), _ptr(out_r), _ptr(delta), _stream_ptr(stream)),
               "cuot_ferret_run")
 
+    def ferret_run_noalloc(self, cfg: FerretConfig, out_s, out_r, delta=None, stream=None):
+        """cuot_ferret_run with the library-held workspace (no allocation in the call)."""
+        check(self.lib.cuot_ferret_run_noalloc(self.handle, C.byref(cfg), _ptr(out_s), _ptr(out_r), _ptr(delta),
+                                               _stream_ptr(stream)),
+              "cuot_ferret_run_noalloc")
+
     def sot_run_stages(self, cfg: SotConfig, ws, v, w, b, delta, stages: int, stream=None):
         check(self.lib.cuot_sot_run_stages(self.handle, C.byref(cfg), _ptr(ws), ws.numel() * ws.element_size(),
                                            _ptr(v), _ptr(w), _ptr(b), _ptr(delta), stages, _stream_ptr(stream)),
@@ -286,6 +292,8 @@ def make_instance_struct(kind: int, n: int, t: int, h: int, k: int, seed, ntasks
 
 SOT_ALL = 127
 FERRET_ALL = 15
+# bytes of the library-held workspace of Context.ferret_run_noalloc (CUOT_FERRET_NOALLOC_WORKSPACE)
+FERRET_NOALLOC_WORKSPACE = 32 << 20
 SOT_STAGES = [("setup", 1), ("qc_prepare", 2), ("ggm_sender", 4), ("ggm_recver", 8), ("qc_sender", 16),
               ("qc_recver", 32), ("sparse", 64)]
 FERRET_STAGES = [("setup", 1), ("ggm_sender", 2), ("ggm_recver", 4), ("lpn", 8)]

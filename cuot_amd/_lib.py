@@ -105,6 +105,7 @@ SIGNATURES = {
     "cuot_sot_dual_lpn": (INT, [P]),
     "cuot_sot_outputs": (INT, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(I64)]),
     "cuot_ferret_run": (INT, [P, C.POINTER(FerretConfig), P, C.c_size_t, P, P, P, P]),
+    "cuot_ferret_run_noalloc": (INT, [P, C.POINTER(FerretConfig), P, P, P, P]),
     "cuot_batch_new": (INT, [P, C.POINTER(Instance), INT, C.c_size_t, C.POINTER(P)]),
     "cuot_batch_workspace_bytes": (C.c_size_t, [P]),
     "cuot_batch_bind": (INT, [P, P, C.c_size_t, C.POINTER(P), C.POINTER(P), C.POINTER(P), P]),

@@ -16,6 +16,7 @@ void cuot_set_last_error(const char* msg, const char* file, int line) {
 }
 
 hipError_t cuot_upload_tables(int device);
+int cuot_noalloc_setup(int device);  // engine.hip: per-device address of the library scratch
 
 extern "C" const char* cuot_last_error(void) { return g_last_error.c_str(); }
 
@@ -26,6 +27,10 @@ extern "C" int cuot_init(int device, cuot_ctx** out) {
   CUOT_CHECK_ARG(device >= 0 && device < n, "device index out of range");
   CUOT_HIP_TRY(hipSetDevice(device));
   CUOT_HIP_TRY(cuot_upload_tables(device));
+  {
+    const int rc = cuot_noalloc_setup(device);
+    if (rc) return rc;
+  }
   cuot_ctx* c = new cuot_ctx();
   c->device = device;
   c->stream = nullptr;  // legacy default stream unless the caller passes one

@@ -11,6 +11,8 @@
 //   MPCOT -> bootstrap COTs (K, M = K ^ c*Delta) -> dual-role LPN.
 #include <string.h>
 
+#include <mutex>
+
 #include "ctx.h"
 #include "engine.h"
 #include "kernels.h"
@@ -370,4 +372,52 @@ extern "C" int cuot_ferret_run_stages(cuot_ctx* ctx, const cuot_ferret_config* c
 extern "C" int cuot_ferret_run(cuot_ctx* ctx, const cuot_ferret_config* cfg, void* ws_d, size_t ws_bytes,
                                cuot_block* out_s_d, cuot_block* out_r_d, cuot_block* delta_d, void* stream) {
   return cuot_ferret_run_stages(ctx, cfg, ws_d, ws_bytes, out_s_d, out_r_d, delta_d, CUOT_FERRET_STAGE_ALL, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Ferret with library-held scratch (no allocation in the call)
+// ---------------------------------------------------------------------------
+// The workspace of cuot_ferret_run lives in this module-scope device array:
+// every GPU the library runs on holds its own copy until the process ends.
+// Host code never uses the array's name as an address: cuot_noalloc_setup
+// (called by cuot_init with the device current) takes each device's address
+// once with hipGetSymbolAddress and every later use goes through that
+// pointer.  All calls on one GPU share the array, so calls that overlap on
+// one GPU race; every run writes each workspace word before it reads it
+// (seed PRG streams, bootstrap, level sums, interleaved kk), so nothing has to
+// start at zero.
+static __device__ __align__(256) unsigned char scratch[CUOT_FERRET_NOALLOC_WORKSPACE];
+static void* g_scratch_dev[64];
+static std::mutex g_scratch_mu;
+
+int cuot_noalloc_setup(int device) {
+  CUOT_CHECK_ARG(device >= 0 && device < 64, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return CUOT_OK;
+  void* p = nullptr;
+  CUOT_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  CUOT_CHECK_ARG(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+  g_scratch_dev[device] = p;
+  return CUOT_OK;
+}
+
+extern "C" int cuot_ferret_run_noalloc(cuot_ctx* ctx, const cuot_ferret_config* cfg, cuot_block* out_s_d,
+                                       cuot_block* out_r_d, cuot_block* delta_d, void* stream) {
+  CUOT_CHECK_ARG(ctx && cfg, "ctx/cfg NULL");
+  FerretPlan p;
+  p.cfg = *cfg;
+  const int rc = p.validate();
+  if (rc) return rc;
+  CUOT_CHECK_ARG(p.carve(nullptr) <= (size_t)CUOT_FERRET_NOALLOC_WORKSPACE,
+                 "workspace of this config (cuot_ferret_workspace_bytes) exceeds the library scratch of "
+                 "CUOT_FERRET_NOALLOC_WORKSPACE bytes");
+  CUOT_CHECK_ARG(ctx->device >= 0 && ctx->device < 64, "device index out of range");
+  void* ws = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    ws = g_scratch_dev[ctx->device];
+  }
+  CUOT_CHECK_ARG(ws != nullptr, "scratch address not set up on this device (cuot_init)");
+  return cuot_ferret_run_stages(ctx, cfg, ws, (size_t)CUOT_FERRET_NOALLOC_WORKSPACE, out_s_d, out_r_d, delta_d,
+                                CUOT_FERRET_STAGE_ALL, stream);
 }

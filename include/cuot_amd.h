@@ -209,6 +209,20 @@ int cuot_ferret_run_stages(cuot_ctx *ctx, const cuot_ferret_config *cfg, void *w
 int cuot_ferret_run(cuot_ctx *ctx, const cuot_ferret_config *cfg, void *ws_d, size_t ws_bytes,
                     cuot_block *out_s_d, cuot_block *out_r_d, cuot_block *delta_d, void *stream);
 
+/* cuot_ferret_run without a workspace argument and without any allocation in
+ * the call: the workspace is a library-held device array of
+ * CUOT_FERRET_NOALLOC_WORKSPACE bytes (32 MiB; enough for n = 10,168,320,
+ * k = 452,160, t = 1,270, h = 13, whose workspace is 28.5 MiB), one per GPU,
+ * held until the process ends.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Configs whose
+ * cuot_ferret_workspace_bytes exceeds the limit return CUOT_ERR_ARG.  Calls
+ * that overlap on one GPU share the array and race: issue them on one stream
+ * or serialise them.  Same outputs as cuot_ferret_run. */
+#define CUOT_FERRET_NOALLOC_WORKSPACE (32u << 20)
+int cuot_ferret_run_noalloc(cuot_ctx *ctx, const cuot_ferret_config *cfg, cuot_block *out_s_d,
+                            cuot_block *out_r_d, cuot_block *delta_d, void *stream);
+
 /* ---- drop-in: emp-ot Ferret dev_layer ----------------------------------------
  * Same argument meaning and blocking, host-in/host-out behaviour as
  * ferret/emp-ot/emp-ot/ferret/dev_layer.h:9-18, with Mat* replaced by arrays

@@ -264,7 +264,7 @@ const CuotKnobs& cuot_knobs() { return g_knobs; }
 
 extern "C" int cuot_set_knobs(const cuot_knob_values* k) {
   CUOT_CHECK_ARG(k, "NULL knobs");
-  CUOT_CHECK_ARG(k->qc_radix == 0 || (k->qc_radix >= 2 && k->qc_radix <= 5), "qc_radix must be 0 or 2..5");
+  CUOT_CHECK_ARG(k->qc_radix == 0 || (k->qc_radix >= 2 && k->qc_radix <= 8), "qc_radix must be 0 or 2..8");
   CUOT_CHECK_ARG(k->qc_cols == 0 || k->qc_cols == 1 || k->qc_cols == 2 || k->qc_cols == 4 || k->qc_cols == 8,
                  "qc_cols must be 0, 1, 2, 4 or 8");
   CUOT_CHECK_ARG(k->ggm_dfs >= 0 && k->ggm_dfs <= 6, "ggm_dfs must be in [0, 6]");

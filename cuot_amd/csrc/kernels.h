@@ -117,7 +117,7 @@ void cuot_rec_end(hipStream_t st);
 // Execution knobs (capi.hip): process-wide, picked by the autotuner; 0 means
 // the built-in rule.  Knobs never change outputs.
 struct CuotKnobs {
-  int qc_radix;  // FFT layers per upper pass (2..5)
+  int qc_radix;  // FFT layers per upper pass (2..8)
   int qc_cols;   // FFT columns per tile (1, 2, 4, 8)
   int ggm_dfs;   // per-lane depth-first levels L (h - 10 <= L <= min(h - 8, 6))
   int lpn_wgs;   // LPN grid cap

@@ -416,6 +416,11 @@ hipError_t launch_conv_rows(unsigned cap, hipStream_t st, uint32_t* Bw, int64_t 
 // Tile: T = 2^R elements spaced 2^lo apart, times C consecutive low columns.
 // Layers k in [klo, khi] (lo <= klo, khi < lo + R).  forward: u ^= c v; v ^= u;
 // inverse: v ^= u; u ^= c v; c = C(idx_u >> (k+1)).
+// NP = 32: a tile element holds all 32 planes of an element.  NP = 16 (half
+// mode, blockIdx.z = the half): it holds one GF(2^16) half, which is enough
+// for layers whose twiddles all lie in the subfield GF(2^16) (k >= m - 16,
+// checked on the host): there the two halves transform independently, so a
+// tile of the same LDS holds twice the elements (more layers per pass).
 enum { PASS_FWD = 0, PASS_INV = 1, PASS_FUSED = 2 };
 
 struct FftPassArgs {
@@ -424,33 +429,66 @@ struct FftPassArgs {
   int lo, R, C, lgC;
   int mode;
   int khi;           // highest layer to apply (FWD/FUSED: start, INV: end)
-  int first;         // load only planes 0..15 of the lower half (top layer input)
+  int first;         // load only planes 0..15 of the lower half (top layer input) and embed the chunks
+  int last;          // completes the inverse transform: unpack the chunk products before the write-back
   const uint32_t* ahat;
   const uint32_t* const* ahats;  // per-instance A^ (batch) or NULL
   int64_t XS_words;  // instance stride of X in 32-bit words (blockIdx.y)
+  // planes 0..15 are read from S0 and written to D0 (X when NULL): a first pass
+  // in half mode cannot overwrite the chunk planes 0..15 that its other half
+  // still reads, so it writes half 0 to D0 = the idle row buffer, from where the
+  // next pass reads it (S0)
+  const uint32_t* S0;
+  uint32_t* D0;
 };
+
+// LDS words per tile element: NP planes x 4 words + 4 pad words (with the pad
+// the element stride is 4 mod 64 banks, so 16 consecutive elements' words of
+// one plane fall on distinct banks)
+template <int NP>
+constexpr int fstride() {
+  return NP * 4 + 4;
+}
+template <int NP>
+constexpr int ftc_max() {
+  return FFT_TC * 32 / NP;  // tile elements for the same LDS footprint
+}
+// half-mode workgroups have twice the lanes (the 16-plane tasks need half the
+// registers): 8 waves each, 4 per SIMD with two workgroups per CU
+template <int NP>
+constexpr int fthreads() {
+  return NP == 16 ? 2 * FFT_THREADS : FFT_THREADS;
+}
+template <int NP>
+constexpr int fld() {  // staged 16-byte words per lane: ftc_max * NP <= fld * fthreads
+  return FFT_LD * FFT_THREADS / fthreads<NP>();
+}
 
 // The top forward layer (k = m - 1, tile-relative s = R - 1) of the first
 // pass: the upper half of the padded input is zero, so u += c*0 leaves u and
 // v += u makes v a copy of u.
+template <int NP>
 __device__ __forceinline__ void fft_copy_top(int R, int C, int lgC, uint32_t* lds) {
+  constexpr int ST = fstride<NP>();
   const int s = R - 1;
   const int ntask = ((1 << R) >> 1) * C * 4;
   for (int tau = threadIdx.x; tau < ntask; tau += blockDim.x) {
     const int w = tau & 3;
     const int c = (tau >> 2) & (C - 1);
     const int bp = (tau >> 2) >> lgC;  // < 2^s: tu = bp, tv = bp + 2^s
-    const uint32_t* pu = lds + fslot(bp * C + c) * FFT_STRIDE + w;
-    uint32_t* pv = lds + fslot((bp | (1 << s)) * C + c) * FFT_STRIDE + w;
+    const uint32_t* pu = lds + fslot(bp * C + c) * ST + w;
+    uint32_t* pv = lds + fslot((bp | (1 << s)) * C + c) * ST + w;
 #pragma unroll
-    for (int p = 0; p < 32; p++) pv[4 * p] = pu[4 * p];
+    for (int p = 0; p < NP; p++) pv[4 * p] = pu[4 * p];
   }
   __syncthreads();
 }
 
-__device__ __forceinline__ void fft_layer(int lo, int R, int C, int lgC, uint32_t* lds, int64_t hib,
-                                                  int64_t cb, int k, bool inv) {
-  const uint32_t* ctab = lds + (C << R) * FFT_STRIDE;
+template <int NP>
+__device__ __forceinline__ void fft_layer(int lo, int R, int C, int lgC, uint32_t* lds, int64_t hib, int64_t cb,
+                                          int k, bool inv) {
+  constexpr int ST = fstride<NP>();
+  const uint32_t* ctab = lds + (C << R) * ST;
   const int s = k - lo;
   const int T = 1 << R;
   const int ntask = (T >> 1) * C * 4;
@@ -462,39 +500,98 @@ __device__ __forceinline__ void fft_layer(int lo, int R, int C, int lgC, uint32_
     const int tv = tu | (1 << s);
     const int64_t idx = (hib << (lo + R)) | ((int64_t)tu << lo) | (cb * C + c);
     const uint64_t g = (uint64_t)idx >> (k + 1);
-    uint32_t* pu = lds + fslot(tu * C + c) * FFT_STRIDE + w;
-    uint32_t* pv = lds + fslot(tv * C + c) * FFT_STRIDE + w;
+    uint32_t* pu = lds + fslot(tu * C + c) * ST + w;
+    uint32_t* pv = lds + fslot(tv * C + c) * ST + w;
     // forward: u += c*v, v += u; inverse: v += u, u += c*v.  Only v (or the
     // new v) and the product are held in registers; u streams through LDS.
-    uint32_t xv[32], acc[32];
+    uint32_t xv[NP], acc[NP];
     if (inv) {
 #pragma unroll
-      for (int p = 0; p < 32; p++) {
+      for (int p = 0; p < NP; p++) {
         xv[p] = pv[4 * p] ^ pu[4 * p];
         pv[4 * p] = xv[p];
       }
     } else {
 #pragma unroll
-      for (int p = 0; p < 32; p++) xv[p] = pv[4 * p];
+      for (int p = 0; p < NP; p++) xv[p] = pv[4 * p];
     }
     const uint32_t g0 = __builtin_amdgcn_readfirstlane((uint32_t)g);
-    if (__all((uint32_t)g == g0)) {
-      const uint32_t cst = __builtin_amdgcn_readfirstlane(ctab_const(ctab, g0));
-      gf32_mulc_top<true>(xv, cst, acc);
+    if constexpr (NP == 16) {
+      // one GF(2^16) half, twiddle in GF(2^16)
+      if (__all((uint32_t)g == g0))
+        gf16_mulc(xv, __builtin_amdgcn_readfirstlane(ctab_const(ctab, g0)), acc);
+      else
+        gf16_mulc(xv, ctab_const(ctab, (uint32_t)g), acc);
     } else {
-      gf32_mulc_top<false>(xv, ctab_const(ctab, (uint32_t)g), acc);
+      // the twiddle lies in the subfield GF(2^16) (high half zero) for every
+      // layer k >= m - 16: two 16-bit products instead of one 32-bit product
+      if (__all((uint32_t)g == g0)) {
+        const uint32_t cst = __builtin_amdgcn_readfirstlane(ctab_const(ctab, g0));
+        if ((cst >> 16) == 0u)
+          gf32_mulc_sub<true>(xv, cst, acc);
+        else
+          gf32_mulc<true>(xv, cst, acc);
+      } else {
+        const uint32_t cst = ctab_const(ctab, (uint32_t)g);
+        if (__all((cst >> 16) == 0u))
+          gf32_mulc_sub<false>(xv, cst, acc);
+        else
+          gf32_mulc<false>(xv, cst, acc);
+      }
     }
     if (inv) {
 #pragma unroll
-      for (int p = 0; p < 32; p++) pu[4 * p] ^= acc[p];
+      for (int p = 0; p < NP; p++) pu[4 * p] ^= acc[p];
     } else {
 #pragma unroll
-      for (int p = 0; p < 32; p++) {
+      for (int p = 0; p < NP; p++) {
         const uint32_t un = pu[4 * p] ^ acc[p];
         pu[4 * p] = un;
         pv[4 * p] = xv[p] ^ un;
       }
     }
+  }
+  __syncthreads();
+}
+
+// Chunk embedding sweeps over the staged tile (element-independent linear maps
+// on the planes of each element word): the 16 chunk-bit planes become the
+// coordinates of u(theta) (all 32, or the 16 of half h) before the first
+// forward layer, and after the last inverse layer the 32 coordinates of a chunk
+// product become its 31 coefficient planes (plane 31 zero), which the inverse
+// conversion and the fold read.
+template <int NP>
+__device__ __forceinline__ void fft_pack_tile(int TC, int h, uint32_t* lds) {
+  constexpr int ST = fstride<NP>();
+  for (int tau = threadIdx.x; tau < TC * 4; tau += blockDim.x) {
+    uint32_t* pe = lds + fslot(tau >> 2) * ST + (tau & 3);
+    uint32_t in[16], out[NP];
+#pragma unroll
+    for (int p = 0; p < 16; p++) in[p] = pe[4 * p];
+    if constexpr (NP == 32) {
+      gf32_pack_planes(in, out);
+    } else {
+      if (h)
+        gf32_pack_planes_half<16>(in, out);
+      else
+        gf32_pack_planes_half<0>(in, out);
+    }
+#pragma unroll
+    for (int p = 0; p < NP; p++) pe[4 * p] = out[p];
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void fft_unpack_tile(int TC, uint32_t* lds) {
+  for (int tau = threadIdx.x; tau < TC * 4; tau += blockDim.x) {
+    uint32_t* pe = lds + fslot(tau >> 2) * FFT_STRIDE + (tau & 3);
+    uint32_t in[32], out[31];
+#pragma unroll
+    for (int p = 0; p < 32; p++) in[p] = pe[4 * p];
+    gf32_unpack_planes(in, out);
+#pragma unroll
+    for (int p = 0; p < 31; p++) pe[4 * p] = out[p];
+    pe[4 * 31] = 0u;
   }
   __syncthreads();
 }
@@ -513,29 +610,39 @@ __device__ __forceinline__ void fft_pointwise(int lo, int R, int C, int lgC,
     uint32_t x[32], y[32];
 #pragma unroll
     for (int p = 0; p < 32; p++) x[p] = pe[4 * p];
-    gf32_mulc_top<false>(x, ahat[idx], y);
+    gf32_mulc<false>(x, ahat[idx], y);
 #pragma unroll
     for (int p = 0; p < 32; p++) pe[4 * p] = y[p];
   }
   __syncthreads();
 }
 
-__global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void qc_fft_pass(FftPassArgs a) {
+template <int NP>
+__global__ __launch_bounds__(fthreads<NP>()) __attribute__((amdgpu_waves_per_eu(NP == 16 ? 4 : 2))) void qc_fft_pass(
+    FftPassArgs a) {
   extern __shared__ uint32_t lds[];
-  a.X += (int64_t)blockIdx.y * a.XS_words;
+  constexpr int ST = fstride<NP>();
+  constexpr int NT = fthreads<NP>();
+  constexpr int LD = fld<NP>();
+  const int64_t ioff = (int64_t)blockIdx.y * a.XS_words;
+  a.X += ioff;
+  const uint4* S0 = (const uint4*)(a.S0 ? a.S0 + ioff : a.X);
+  uint4* D0 = (uint4*)(a.D0 ? a.D0 + ioff : a.X);
   if (a.ahats) a.ahat = a.ahats[blockIdx.y];
+  const int h = NP == 16 ? (int)blockIdx.z : 0;  // half of a half-mode pass
+  const int pbase = 16 * h;
   const int T = 1 << a.R;
   const int TC = T * a.C;
   const int64_t cblocks = ((int64_t)1 << a.lo) >> a.lgC;
   const int64_t ntiles = a.M / TC;
   const int64_t half = a.M >> 1;
-  ctab_fill(lds + TC * FFT_STRIDE, threadIdx.x, blockDim.x);  // read after the first barrier
+  ctab_fill(lds + TC * ST, threadIdx.x, blockDim.x);  // read after the first barrier
   // Each lane stages up to FFT_LD 16-byte plane words of a tile (f = (p*T +
-  // t)*C + c: c, t fastest, contiguous runs; TC * 32 <= FFT_LD * blockDim).
+  // t)*C + c: c, t fastest, contiguous runs; TC * NP <= FFT_LD * blockDim).
   // Two WGs share a CU, so one's loads and stores overlap the other's layers.
-  const int n4 = TC * 32;
+  const int n4 = TC * NP;
   const uint4* X4 = (const uint4*)a.X;
-  uint4 v[FFT_LD];
+  uint4 v[LD];
   // staging addresses come from an opaque copy of the lane index, so that the
   // compiler recomputes them per phase instead of holding them through the
   // layers
@@ -544,19 +651,24 @@ __global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(2))
     asm volatile("" : "+v"(t));
     return t;
   };
+  // the first pass reads the 16 chunk planes (lower half only) whatever the half
   auto load_tile = [&](int64_t tl) {
     const int64_t lcb = tl & (cblocks - 1), lhib = tl >> (a.lo - a.lgC);
     const int tid = opaque_tid();
 #pragma unroll
-    for (int u = 0; u < FFT_LD; u++) {
-      const int f = tid + u * FFT_THREADS;
+    for (int u = 0; u < LD; u++) {
+      const int f = tid + u * NT;
       v[u] = z4();
       if (f < n4) {
         const int c = f & (a.C - 1);
         const int pt = f >> a.lgC;
         const int t = pt & (T - 1), p = pt >> a.R;
         const int64_t idx = (lhib << (a.lo + a.R)) | ((int64_t)t << a.lo) | (lcb * a.C + c);
-        if (!a.first || (idx < half && p < 16)) v[u] = X4[(int64_t)p * a.M + idx];
+        const int gp = pbase + p;  // global plane
+        if (!a.first)
+          v[u] = (gp < 16 ? S0 : X4)[(int64_t)gp * a.M + idx];
+        else if (idx < half && p < 16)
+          v[u] = X4[(int64_t)p * a.M + idx];
       }
     }
   };
@@ -572,16 +684,17 @@ __global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(2))
   for (int64_t tile = slot; tile < ntiles; tile += G) {
     const int64_t cb = tile & (cblocks - 1), hib = tile >> (a.lo - a.lgC);
 #pragma unroll
-    for (int u = 0; u < FFT_LD; u++) {
+    for (int u = 0; u < LD; u++) {
       const int f = (int)threadIdx.x + u * (int)blockDim.x;
       if (f < n4) {
         const int c = f & (a.C - 1);
         const int pt = f >> a.lgC;
         const int t = pt & (T - 1), p = pt >> a.R;
-        *(uint4*)&lds[fslot(t * a.C + c) * FFT_STRIDE + p * 4] = v[u];
+        *(uint4*)&lds[fslot(t * a.C + c) * ST + p * 4] = v[u];
       }
     }
     __syncthreads();
+    if (a.first) fft_pack_tile<NP>(TC, h, lds);
     // one step loop so that the layer body is instantiated once: forward
     // layers khi..lo, the pointwise product (fused pass), inverse layers lo..khi
     const int nl = a.khi - a.lo + 1;
@@ -590,40 +703,44 @@ __global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(2))
     const int s_inv = (a.mode == PASS_INV || a.mode == PASS_FUSED) ? nl : 0;
     for (int step = 0; step < s_fwd + s_pw + s_inv; step++) {
       if (step == s_fwd && s_pw) {
-        fft_pointwise(a.lo, a.R, a.C, a.lgC, a.ahat, lds, hib, cb);
+        if constexpr (NP == 32) fft_pointwise(a.lo, a.R, a.C, a.lgC, a.ahat, lds, hib, cb);
       } else {
         const bool inv = step >= s_fwd;
         const int k = inv ? a.lo + (step - s_fwd - s_pw) : a.khi - step;
         if (a.first && step == 0)
-          fft_copy_top(a.R, a.C, a.lgC, lds);  // top layer on a zero upper half: v = u
+          fft_copy_top<NP>(a.R, a.C, a.lgC, lds);  // top layer on a zero upper half: v = u
         else
-          fft_layer(a.lo, a.R, a.C, a.lgC, lds, hib, cb, k, inv);
+          fft_layer<NP>(a.lo, a.R, a.C, a.lgC, lds, hib, cb, k, inv);
       }
     }
-    uint4 w[FFT_LD];
+    if constexpr (NP == 32) {
+      if (a.last) fft_unpack_tile(TC, lds);
+    }
+    uint4 w[LD];
     const int tid = opaque_tid();
 #pragma unroll
-    for (int u = 0; u < FFT_LD; u++) {
-      const int f = tid + u * FFT_THREADS;
+    for (int u = 0; u < LD; u++) {
+      const int f = tid + u * NT;
       w[u] = z4();
       if (f < n4) {
         const int c = f & (a.C - 1);
         const int pt = f >> a.lgC;
         const int t = pt & (T - 1);
-        w[u] = *(const uint4*)&lds[fslot(t * a.C + c) * FFT_STRIDE + (pt >> a.R) * 4];
+        w[u] = *(const uint4*)&lds[fslot(t * a.C + c) * ST + (pt >> a.R) * 4];
       }
     }
     if (tile + G < ntiles) load_tile(tile + G);
     uint4* Y4 = (uint4*)a.X;
 #pragma unroll
-    for (int u = 0; u < FFT_LD; u++) {
-      const int f = tid + u * FFT_THREADS;
+    for (int u = 0; u < LD; u++) {
+      const int f = tid + u * NT;
       if (f < n4) {
         const int c = f & (a.C - 1);
         const int pt = f >> a.lgC;
         const int t = pt & (T - 1), p = pt >> a.R;
         const int64_t idx = (hib << (a.lo + a.R)) | ((int64_t)t << a.lo) | (cb * a.C + c);
-        Y4[(int64_t)p * a.M + idx] = w[u];
+        const int gp = pbase + p;
+        (gp < 16 ? D0 : Y4)[(int64_t)gp * a.M + idx] = w[u];
       }
     }
     __syncthreads();
@@ -634,14 +751,7 @@ __global__ __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(2))
 // ---- scalar transform of the single polynomial a (A^ = its evaluations) ------
 // Same conversion + FFT as the bitsliced path, on one 32-bit field element per
 // index instead of 128 bitsliced rows (A^ is computed once per code).
-CUOT_DEV uint32_t gf32_mul_dev(uint32_t a, uint32_t b) {
-  uint64_t r = 0;
-  for (int i = 0; i < 32; i++)
-    if ((b >> i) & 1u) r ^= (uint64_t)a << i;
-  for (int k = 62; k >= 32; k--)
-    if ((r >> k) & 1ull) r ^= ((uint64_t)CUOT_GF_POLY_LOW | (1ull << 32)) << (k - 32);
-  return (uint32_t)r;
-}
+CUOT_DEV uint32_t gf32_mul_dev(uint32_t a, uint32_t b) { return gf32_mul(a, b); }
 
 __global__ void qcs_pack(const QcPrep* __restrict__ pr, QcPrep pr1, int64_t M) {
   const QcPrep q = pr ? pr[blockIdx.y] : pr1;
@@ -656,7 +766,7 @@ __global__ void qcs_pack(const QcPrep* __restrict__ pr, QcPrep pr1, int64_t M) {
         if (src < n) v |= (uint32_t)((a[src >> 3] >> (src & 7)) & 1u) << b;
       }
     }
-    A[c] = v;
+    A[c] = gf32_pack_chunk(v);
   }
 }
 
@@ -727,7 +837,7 @@ __global__ __launch_bounds__(1024) void qcs_prepare_lds(const QcPrep* __restrict
         if (src < q.n) v |= (uint32_t)((q.a_bits[src >> 3] >> (src & 7)) & 1u) << b;
       }
     }
-    L[c] = v;
+    L[c] = gf32_pack_chunk(v);
   }
   __syncthreads();
   // monomial -> novel basis on the lower half (division by s_k, k = m-2 .. 0)
@@ -1006,8 +1116,11 @@ hipError_t configure_lds() {
     hipError_t e0 = hipMemcpyToSymbol(HIP_SYMBOL(c_beta), beta().v, sizeof(uint32_t) * 32);
     if (e0 != hipSuccess) return e0;
   }
-  hipError_t e = hipFuncSetAttribute((const void*)qc_fft_pass, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (FFT_TC * FFT_STRIDE + CTAB_WORDS) * 4);
+  hipError_t e = hipFuncSetAttribute((const void*)qc_fft_pass<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (ftc_max<32>() * fstride<32>() + CTAB_WORDS) * 4);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)qc_fft_pass<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (ftc_max<16>() * fstride<16>() + CTAB_WORDS) * 4);
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute((const void*)qcs_prepare_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
                           ((int64_t)1 << PREP_LDS_MMAX) * 4);
@@ -1139,13 +1252,13 @@ hipError_t conv_inverse(Bufs b, int64_t M, int P, int mtop, bool fold, hipStream
 }
 
 // two resident WGs per CU; every WG walks many tiles
-void pass_grid(int64_t ntiles, int B, unsigned& g) {
-  int64_t cap = FFT_WGS / (B < 1 ? 1 : B);
+void pass_grid(int64_t ntiles, int B, int Z, unsigned& g) {
+  int64_t cap = FFT_WGS / ((B < 1 ? 1 : B) * Z);
   if (cap < 1) cap = 1;
   g = (unsigned)(ntiles < cap ? ntiles : cap);
 }
 
-FftPassArgs pass_args(const Bufs& b, int64_t M, int lo, int R, int C, int mode, int khi, int first,
+FftPassArgs pass_args(const Bufs& b, int64_t M, int lo, int R, int C, int mode, int khi, int first, int last,
                       const uint32_t* ahat1, const uint32_t* const* ahats) {
   FftPassArgs a;
   a.X = b.X;
@@ -1157,23 +1270,77 @@ FftPassArgs pass_args(const Bufs& b, int64_t M, int lo, int R, int C, int mode, 
   a.mode = mode;
   a.khi = khi;
   a.first = first;
+  a.last = last;
   a.ahat = ahat1;
   a.ahats = ahats;
   a.XS_words = b.XS * 4;
+  a.S0 = nullptr;
+  a.D0 = nullptr;
   return a;
 }
 
+template <int NP>
 void launch_pass(const FftPassArgs& a, int B, hipStream_t st) {
+  constexpr int Z = 32 / NP;  // halves per element handled by separate workgroups
   unsigned g;
-  pass_grid(a.M / ((1 << a.R) * a.C), B, g);
+  pass_grid(a.M / ((1 << a.R) * a.C), B, Z, g);
   // algorithmic bytes: every element's 32 planes read and written once (the
-  // first pass reads only planes 0..15 of the lower half)
+  // first pass reads only planes 0..15 of the lower half, once per half)
   const double per = (double)a.M * 512.0 * B;
-  cuot_rec_begin(st, "qc_fft_pass", a.first ? per * 1.25 : per * 2.0);
-  hipLaunchKernelGGL(qc_fft_pass, dim3(g, (unsigned)B), dim3(FFT_THREADS),
-                     ((size_t)(1 << a.R) * a.C * FFT_STRIDE + CTAB_WORDS) * 4,
-                     st, a);
+  cuot_rec_begin(st, "qc_fft_pass", a.first ? per * (NP == 16 ? 1.5 : 1.25) : per * 2.0);
+  hipLaunchKernelGGL(qc_fft_pass<NP>, dim3(g, (unsigned)B, (unsigned)Z), dim3(fthreads<NP>()),
+                     ((size_t)(1 << a.R) * a.C * fstride<NP>() + CTAB_WORDS) * 4, st, a);
   cuot_rec_end(st);
+}
+
+// Shape of an upper pass over layers [lo, lo + R): half mode when every layer
+// has its twiddles in GF(2^16) (lo >= m - 16); R layers (knob qc_radix, else 7
+// in half mode and 5 otherwise), C columns (knob qc_cols, else as many as the
+// tile holds, at most the column block 2^lo).
+struct PassShape {
+  int lo, R, C;
+  bool half;
+};
+
+PassShape upper_shape(int m, int kfrom, int kto, bool down) {
+  // kfrom..kto: remaining layers (down: kfrom >= kto, taken from the top)
+  const int nrem = down ? kfrom - kto + 1 : kto - kfrom + 1;
+  const int knobR = cuot_knobs().qc_radix;
+  PassShape ps;
+  for (int pass_half = 1; pass_half >= 0; pass_half--) {
+    int R = knobR > 0 ? knobR : (pass_half ? 7 : 5);
+    if (R > nrem) R = nrem;
+    const int lo = down ? kfrom - R + 1 : kfrom;
+    if (pass_half && lo < m - 16) continue;
+    const int tcmax = pass_half ? ftc_max<16>() : ftc_max<32>();
+    while ((1 << R) > tcmax) R--;
+    ps.R = R;
+    ps.lo = down ? kfrom - R + 1 : kfrom;
+    ps.half = pass_half != 0;
+    int C = cuot_knobs().qc_cols > 0 ? cuot_knobs().qc_cols : FFT_TC;
+    while (C > 1 && (((int64_t)1 << ps.lo) % C)) C >>= 1;
+    while (((1 << R) * C) > tcmax) C >>= 1;
+    ps.C = C;
+    return ps;
+  }
+  return ps;
+}
+
+// unpack of the chunk products after a last inverse pass in half mode: the 32
+// coordinates of every element word become its 31 coefficient planes (plane 31
+// zero).  Plane p of instance i is words [(i * 32 + p) * 4M, ... + 4M).
+__global__ __launch_bounds__(256) void qc_unpack_x(uint32_t* __restrict__ X, int64_t M, int64_t XS_words) {
+  uint32_t* Xi = X + (int64_t)blockIdx.y * XS_words;
+  const int64_t pw = M * 4;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < pw; t += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t in[32], out[31];
+#pragma unroll
+    for (int p = 0; p < 32; p++) in[p] = Xi[p * pw + t];
+    gf32_unpack_planes(in, out);
+#pragma unroll
+    for (int p = 0; p < 31; p++) Xi[p * pw + t] = out[p];
+    Xi[31 * pw + t] = 0u;
+  }
 }
 
 // forward FFT layers m-1..0 (top layer from the packed lower half); with A^,
@@ -1185,19 +1352,25 @@ hipError_t fft_forward(const Bufs& b, int m, const uint32_t* ahat1, const uint32
   const int R0 = m < FFT_LOGTC ? m : FFT_LOGTC;
   int khi = m - 1;
   bool first = true;
-  const int RMAX = cuot_knobs().qc_radix > 0 ? cuot_knobs().qc_radix : 5;
-  const int CMAX = cuot_knobs().qc_cols > 0 ? cuot_knobs().qc_cols : FFT_TC;
+  bool h0_in_rows = false;  // half 0 of the previous (first, half-mode) pass sits in the row buffer
   while (khi >= R0) {
-    int R = khi - R0 + 1 < RMAX ? khi - R0 + 1 : RMAX;
-    int lo = khi - R + 1;
-    int C = CMAX;
-    while (C > 1 && (((int64_t)1 << lo) % C)) C >>= 1;
-    while (((1 << R) * C) > FFT_TC) C >>= 1;
-    launch_pass(pass_args(b, M, lo, R, C, PASS_FWD, khi, first ? 1 : 0, nullptr, nullptr), b.B, st);
+    const PassShape ps = upper_shape(m, khi, R0, true);
+    FftPassArgs a = pass_args(b, M, ps.lo, ps.R, ps.C, PASS_FWD, khi, first ? 1 : 0, 0, nullptr, nullptr);
+    if (h0_in_rows) a.S0 = b.Bw;
+    h0_in_rows = first && ps.half;
+    if (h0_in_rows) a.D0 = b.Bw;
+    if (ps.half)
+      launch_pass<16>(a, b.B, st);
+    else
+      launch_pass<32>(a, b.B, st);
     first = false;
-    khi = lo - 1;
+    khi = ps.lo - 1;
   }
-  launch_pass(pass_args(b, M, 0, R0, 1, fused ? PASS_FUSED : PASS_FWD, R0 - 1, first ? 1 : 0, ahat1, ahats), b.B, st);
+  // the fused pass completes the inverse transform when no upper layers exist
+  FftPassArgs a = pass_args(b, M, 0, R0, 1, fused ? PASS_FUSED : PASS_FWD, R0 - 1, first ? 1 : 0,
+                            (fused && R0 == m) ? 1 : 0, ahat1, ahats);
+  if (h0_in_rows) a.S0 = b.Bw;
+  launch_pass<32>(a, b.B, st);
   *low_khi = R0 - 1;
   return hipGetLastError();
 }
@@ -1205,15 +1378,23 @@ hipError_t fft_forward(const Bufs& b, int m, const uint32_t* ahat1, const uint32
 hipError_t fft_inverse_upper(const Bufs& b, int m, int from, hipStream_t st) {
   const int64_t M = (int64_t)1 << m;
   int klo = from;
-  const int RMAX = cuot_knobs().qc_radix > 0 ? cuot_knobs().qc_radix : 5;
-  const int CMAX = cuot_knobs().qc_cols > 0 ? cuot_knobs().qc_cols : FFT_TC;
   while (klo <= m - 1) {
-    int R = m - klo < RMAX ? m - klo : RMAX;
-    int C = CMAX;
-    while (C > 1 && (((int64_t)1 << klo) % C)) C >>= 1;
-    while (((1 << R) * C) > FFT_TC) C >>= 1;
-    launch_pass(pass_args(b, M, klo, R, C, PASS_INV, klo + R - 1, 0, nullptr, nullptr), b.B, st);
-    klo += R;
+    const PassShape ps = upper_shape(m, klo, m - 1, false);
+    const bool last = ps.lo + ps.R == m;
+    FftPassArgs a = pass_args(b, M, ps.lo, ps.R, ps.C, PASS_INV, ps.lo + ps.R - 1, 0, (last && !ps.half) ? 1 : 0,
+                              nullptr, nullptr);
+    if (ps.half) {
+      launch_pass<16>(a, b.B, st);
+      if (last) {
+        cuot_rec_begin(st, "qc_unpack_x", 2.0 * 512.0 * (double)M * b.B);
+        hipLaunchKernelGGL(qc_unpack_x, dim3(capg(M * 4 / 256, 8192), (unsigned)b.B), dim3(256), 0, st, b.X, M,
+                           b.XS * 4);
+        cuot_rec_end(st);
+      }
+    } else {
+      launch_pass<32>(a, b.B, st);
+    }
+    klo += ps.R;
   }
   return hipGetLastError();
 }

@@ -404,7 +404,7 @@ int cuot_mlp_adam(cuot_ctx *ctx, float *params_d, const float *grad_d, float *m_
  * by the autotuner (cuot_amd.tune); 0 selects the built-in rule.  Knobs never
  * change outputs.  Set them while no work is being launched. */
 typedef struct {
-  int32_t qc_radix; /* FFT layers per upper pass, 2..5 */
+  int32_t qc_radix; /* FFT layers per upper pass, 2..8 (tiles of one GF(2^16) half hold 256 elements, full ones 128) */
   int32_t qc_cols;  /* FFT columns per tile: 1, 2, 4 or 8 */
   int32_t ggm_dfs;  /* GGM per-lane depth-first levels (valid: h-10 <= L <= min(h-8, 6); else the rule) */
   int32_t lpn_wgs;  /* LPN grid cap */

@@ -19,12 +19,13 @@ def test_conversion_host_build():
 
 
 def test_gf32_bitsliced_multiply_host_build():
-    """gf32_mulc_top (uniform two-bit and one-bit steps, per-lane steps) equals
-    scalar GF(2^32) multiplication on every row."""
+    """The tower GF(2^32) of the QC encoder is a field (axioms, x^(2^32-1) = 1,
+    subfield and theta degree), its chunk embedding reproduces carry-less chunk
+    products, its Cantor basis satisfies beta_i^2 + beta_i = beta_{i-1}, and the
+    bitsliced products (subfield and general constants, wave-uniform and
+    per-lane) and plane maps equal the scalar ones on every row."""
     with tempfile.TemporaryDirectory() as d:
-        for two_bit in (0, 1):
-            exe = os.path.join(d, f"gf32_check{two_bit}")
-            subprocess.run(["g++", "-O2", "-std=c++17", f"-DGF32_TWO_BIT={two_bit}", "-o", exe,
-                            os.path.join(HERE, "gf32_host_check.cpp")], check=True)
-            out = subprocess.run([exe], check=True, capture_output=True, text=True, timeout=120).stdout
-            assert "gf32 top ok" in out
+        exe = os.path.join(d, "gf32_check")
+        subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(HERE, "gf32_host_check.cpp")], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True, timeout=300).stdout
+        assert "gf32 tower ok" in out

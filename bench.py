@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-sweep", type=str, default="50,100,200,400,800",
+                    help="offered loads (req/s) of the serving sweep; empty to skip")
+    ap.add_argument("--sweep-requests", type=int, default=300)
+    ap.add_argument("--batch-first", action="store_true", help="run the batched leg before the serving leg")
     ap.add_argument("--qc-only", action="store_true",
                     help="time only the quasi-cyclic encode of one party at n = 2^sot-n-log (after a 256 MiB "
                          "device copy that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE) and print one JSON line")
@@ -93,31 +97,60 @@ def qc_only(args):
     return 0
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(threads: int):
-    """Oracle SOT at n = 2^20 (t = 256, h = 13): 1 warm-up + median of 3."""
+    """The oracle (C, OpenMP; AES-NI, CLMUL) on the host cores, following the
+    reference's timing protocol (silent/main.cu:96-101: warm-up, then repeated
+    runs): SOT at n = 2^20 (t = 256, h = 13, the CPU-path workload), median of
+    5 runs after 1 warm-up with `threads` threads plus one single-thread run;
+    the Ferret instance n = 10,168,320 (k = 452,160, t = 1,270) the same way."""
     import numpy as np
 
     from oracle import pyoracle as po
 
-    n, t, h = 1 << 20, 256, 13
     k0 = np.array(SOT_K0, dtype=np.uint64)
     k1 = np.array(SOT_K1, dtype=np.uint64)
     seed = np.array(SEED, dtype=np.uint64)
+
+    def med(fn, reps):
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t0)
+        times.sort()
+        return times[len(times) // 2]
+
+    n, t, h = 1 << 20, 256, 13
     po.sot_run(1 << 14, 16, 11, k0, k1, seed, nthreads=threads)
-    times = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        po.sot_run(n, t, h, k0, k1, seed, nthreads=threads)
-        times.append(time.perf_counter() - t0)
-    times.sort()
-    med = times[len(times) // 2]
+    sot_all = med(lambda: po.sot_run(n, t, h, k0, k1, seed, nthreads=threads), 5)
+    sot_one = med(lambda: po.sot_run(n, t, h, k0, k1, seed, nthreads=1), 1)
+    fn, fk, ft, fh = 10168320, 452160, 1270, 13
+    po.ferret_run(20000, 1000, 3, 13, seed, nthreads=threads)
+    fer_all = med(lambda: po.ferret_run(fn, fk, ft, fh, seed, nthreads=threads), 5)
+    fer_one = med(lambda: po.ferret_run(fn, fk, ft, fh, seed, nthreads=1), 1)
     return {
-        "value": round(n / med, 1),
+        "value": round(n / sot_all, 1),
         "unit": "COTs/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle SOT n=2^20 t=256 h=13 both roles + QC, median of 3 runs after 1 warm-up "
-                  f"({med:.2f} s/run)",
+        "sample": f"oracle SOT n=2^20 t=256 h=13 both roles + QC, median of 5 runs after 1 warm-up on {threads} "
+                  f"threads ({sot_all:.2f} s/run); one thread {sot_one:.2f} s/run",
+        "one_thread": round(n / sot_one, 1),
+        "ferret": {"n": fn, "value": round(fn / fer_all, 1), "one_thread": round(fn / fer_one, 1),
+                   "unit": "COTs/s", "s_per_run": round(fer_all, 3)},
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
+        "affinity_cores": len(os.sched_getaffinity(0)),
     }
 
 
@@ -137,12 +170,44 @@ def measured_traffic(kernel: str):
     return None
 
 
+def tune_for_mix(ctx, mix):
+    """Execution knobs for the batched mix from the project's autotuner: QC
+    encode and GGM times measured on a grid of sizes and knob settings, the
+    learned cost model trained on them, and the knobs it predicts fastest for
+    the SOT size class that holds the most COTs (knobs are process-wide).
+    Returns (knobs, tuning seconds)."""
+    from cuot_amd import tune
+    from cuot_amd.workloads import SOT
+
+    t0 = time.perf_counter()
+    sots = [wi for wi in mix if wi.kind == SOT]
+    work = {}
+    for wi in sots:
+        work[wi.n.bit_length()] = work.get(wi.n.bit_length(), 0) + wi.n
+    lead = max(work, key=work.get)
+    n_lead = max(wi.n for wi in sots if wi.n.bit_length() == lead)
+    tu = tune.Autotuner(ctx, reps=2)
+    tu.collect_qc([1 << 18, 1 << 20, 1 << 21, n_lead])
+    tu.train("qc", epochs=800)
+    lead_inst = max((wi for wi in sots if wi.n.bit_length() == lead), key=lambda w: w.n)
+    shapes = sorted({(wi.t, wi.h) for wi in sots})[-3:]
+    tu.collect_ggm(shapes)
+    tu.train("ggm", epochs=800)
+    knobs = dict(tu.choose_qc(n_lead))
+    knobs.update(tu.choose_ggm(lead_inst.t, lead_inst.h))
+    tune.set_knobs(**knobs)
+    return knobs, time.perf_counter() - t0
+
+
 def batch_leg(torch, cuot_amd, ctx, steps):
     """64 instances (log-uniform 2^16..2^22, SOT/Ferret alternating): plain
-    one-instance-at-a-time runs vs the batched engine replayed as a graph."""
+    one-instance-at-a-time runs vs the batched engine replayed as a graph, both
+    with the knobs the autotuner picks for the mix."""
     from cuot_amd.workloads import SOT, batch_mix
 
     mix = batch_mix(64)
+    knobs, tune_s = tune_for_mix(ctx, mix)
+    knobs = dict(knobs, tuning_s=round(tune_s, 1))
     insts = [cuot_amd.make_instance_struct(wi.kind, wi.n, wi.t, wi.h, wi.k, wi.seed, k0=SOT_K0, k1=SOT_K1)
              for wi in mix]
     total = sum(wi.n for wi in mix)
@@ -189,6 +254,8 @@ def batch_leg(torch, cuot_amd, ctx, steps):
 
     plain_ms = timed(run_plain)
     batch_ms = timed(run_batch)
+    # the same launch sequence without graph capture: the launch gaps the graph removes
+    eager_ms = timed(lambda: bt.run(graph=False, stream=stream, overlap=True))
     # where the plain form spends its time: per-instance events, by kind and size
     evs = []
     for item in plain:
@@ -209,6 +276,20 @@ def batch_leg(torch, cuot_amd, ctx, steps):
         ent["instances"] += 1
         ent["cots"] += wi.n
         ent["ms"] = round(ent["ms"] + e0.elapsed_time(e1), 4)
+    # the batched form per size class: each class alone as one batch (graph replay)
+    classes = {}
+    for i, wi in enumerate(mix):
+        key = f"{'sot' if wi.kind == SOT else 'ferret'}_log2n_{int(wi.n).bit_length() - 1}"
+        classes.setdefault(key, []).append(i)
+    batched_by_size = {}
+    for key, idx in sorted(classes.items()):
+        sub = cuot_amd.Batch(ctx, [insts[i] for i in idx])
+        ms = timed(lambda: sub.run(graph=True, stream=stream, overlap=True))
+        cots = sum(mix[i].n for i in idx)
+        batched_by_size[key] = {"instances": len(idx), "cots": cots, "ms": round(ms, 4),
+                                "value": round(cots / (ms / 1e3), 1),
+                                "plain_value": round(cots / (by_size[key]["ms"] / 1e3), 1)}
+        sub.close()
     exact = True
     for i, (wi, cfg, ws, o0, o1, b, d) in enumerate(plain):
         exact &= bool(torch.equal(o0, bt.out0[i])) and bool(torch.equal(o1, bt.out1[i]))
@@ -223,17 +304,52 @@ def batch_leg(torch, cuot_amd, ctx, steps):
                     "form": "one launch sequence per stage over all instances, HIP graph replay, "
                             "preparation, sparse encodes, LPN and half the QC chunks on a second stream"},
         "speedup": round(plain_ms / batch_ms, 3),
+        "eager_ms": round(eager_ms, 3),
+        "graph_saves_ms": round(eager_ms - batch_ms, 3),
         "plain_by_size": dict(sorted(by_size.items())),
+        "batched_by_size": batched_by_size,
+        "knobs": knobs,
         "byte_exact_vs_plain": exact,
     }
     bt.close()
     del plain
+    from cuot_amd import tune
+
+    tune.set_knobs()
     return out
 
 
-def serve_leg(torch, cuot_amd, ctx, count, rate, budget_gb):
+def serve_leg(torch, cuot_amd, ctx, count, rate, budget_gb, sweep_rates=(), sweep_count=300, arena=None):
     """Poisson arrivals, n = floor(2^(14 + 10u)), 50/50 SOT/Ferret: continuous
-    batching under an HBM budget vs the FIFO plain loop, real-time replay."""
+    batching under an HBM budget vs the FIFO plain loop, real-time replay; then
+    the same at each offered load of `sweep_rates` with `sweep_count` requests."""
+    import numpy as np
+
+    from cuot_amd.workloads import serving_arrivals
+
+    if arena is None:
+        # one arena of the HBM budget for the whole leg (every server of the
+        # main run and the sweep uses it) instead of one allocation per server
+        torch.cuda.empty_cache()
+        arena = torch.empty(int(budget_gb * (1 << 30)), dtype=torch.uint8, device="cuda")
+    res = serve_run(torch, cuot_amd, ctx, count, rate, budget_gb, arena)
+    sweep = []
+    for r in sweep_rates:
+        row = serve_run(torch, cuot_amd, ctx, sweep_count, r, budget_gb, arena)
+        sweep.append({"rate_hz": r, "requests": sweep_count,
+                      "offered_cots_s": row["config"]["offered_cots_s"],
+                      "served": {k: row["served"][k] for k in ("p50_ms", "p99_ms", "throughput_cots_s",
+                                                               "relation_ok")},
+                      "fifo_plain": {k: row["fifo_plain"][k] for k in ("p50_ms", "p99_ms", "throughput_cots_s",
+                                                                       "relation_ok")}})
+    if sweep:
+        res["sweep"] = sweep
+    del arena
+    torch.cuda.empty_cache()
+    return res
+
+
+def serve_run(torch, cuot_amd, ctx, count, rate, budget_gb, arena=None):
     import numpy as np
 
     from cuot_amd.workloads import serving_arrivals
@@ -244,17 +360,20 @@ def serve_leg(torch, cuot_amd, ctx, count, rate, budget_gb):
     times = [a for a, _ in arr]
     total = sum(wi.n for _, wi in arr)
     budget = int(budget_gb * (1 << 30))
+    span = max(times) - min(times) if len(times) > 1 else 1.0
     res = {"config": {"workload": "serving", "requests": count, "rate_hz": rate,
+                      "offered_cots_s": round(total / span, 1) if span > 0 else None,
                       "sizes": "n = floor(2^(14 + 10u))", "kinds": "SOT / Ferret 50/50", "hbm_budget_bytes": budget,
                       "max_batch": 64, "max_batch_cots": 1 << 22, "streams": 2}}
     torch.cuda.empty_cache()  # the earlier legs' cached blocks go back to the device
+    res["config"]["arena"] = "caller-owned, shared by the servers of the leg" if arena is not None else "server-owned"
     for name, plain in (("served", False), ("fifo_plain", True)):
         torch.cuda.synchronize()
         # batches are capped at 2^22 COTs so that a large request does not hold
         # small ones behind it for a whole 2^24-COT encode
         lat, wall, nb, fails = cuot_amd.serve_replay(ctx, insts, times, hbm_budget=budget, plain=plain,
                                                      max_batch=64, streams=2, max_batch_cots=1 << 22,
-                                                     check_relation=True)
+                                                     check_relation=True, arena=arena)
         res[name] = {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
                      "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
                      "mean_ms": round(float(lat.mean()) * 1e3, 3),
@@ -284,6 +403,7 @@ def main():
     ctx = cuot_amd.Context(0)
     stream = torch.cuda.current_stream()
     B = cuot_amd.to_block
+
 
     # ---- SOT headline -----------------------------------------------------------
     n = 1 << args.sot_n_log
@@ -475,20 +595,27 @@ def main():
     # release the headline buffers before the other legs
     del ws, v, w, b
     torch.cuda.empty_cache()
-    # serving before the batched leg: after the plain half of the batched leg
-    # (64 instance-sized allocations and frees through torch's allocator) the
-    # served p50 doubles in this process while FIFO plain does not change
     serving = None
-    if not args.no_serve:
-        serving = serve_leg(torch, cuot_amd, ctx, args.serve_requests, args.serve_rate, args.serve_budget_gb)
     batched = None
-    if not args.no_batch:
+    rates = [float(r) for r in args.serve_sweep.split(",") if r.strip()]
+    if args.batch_first and not args.no_batch:
+        batched = batch_leg(torch, cuot_amd, ctx, max(1, args.steps))
+        torch.cuda.empty_cache()
+    if not args.no_serve:
+        serving = serve_leg(torch, cuot_amd, ctx, args.serve_requests, args.serve_rate, args.serve_budget_gb, rates,
+                            args.sweep_requests)
+        serving["leg_order"] = "batched leg first" if (args.batch_first and not args.no_batch) else "serving first"
+    if not args.batch_first and not args.no_batch:
         batched = batch_leg(torch, cuot_amd, ctx, max(1, args.steps))
         torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # every core this process may run on, within the CPU share the host grants one GPU (its
+        # OMP_NUM_THREADS) when it sets one
+        share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        cores = len(os.sched_getaffinity(0))
+        threads = args.cpu_threads or (min(cores, share) if share > 0 else cores)
         cpu = cpu_baseline(threads)
 
     value = n / (ms_per_step / 1e3)

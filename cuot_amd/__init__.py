@@ -262,8 +262,9 @@ class Server:
 
 def serve_replay(ctx: Context, instances, arrivals, hbm_budget: int, plain: bool = False, max_batch: int = 0,
                  streams: int = 0, qc_chunk_bytes: int = 0, use_graph: bool = False, max_batch_cots: int = 0,
-                 check_relation: bool = False):
-    """Real-time replay of (arrival seconds, Instance) through a fresh server.
+                 check_relation: bool = False, arena=None):
+    """Real-time replay of (arrival seconds, Instance) through a fresh server
+    (arena: optional caller-owned device tensor of at least hbm_budget bytes).
 
     Returns (latencies in s as numpy array, wall seconds, batches launched,
     relation failures).  With check_relation every result is checked on the
@@ -271,7 +272,10 @@ def serve_replay(ctx: Context, instances, arrivals, hbm_budget: int, plain: bool
     failing indices over all results); otherwise failures is None."""
     n = len(instances)
     cfg = ServerConfig(hbm_budget=hbm_budget, max_batch=max_batch, streams=streams, qc_chunk_bytes=qc_chunk_bytes,
-                       use_graph=int(use_graph), plain=int(plain), max_batch_cots=max_batch_cots)
+                       use_graph=int(use_graph), plain=int(plain), max_batch_cots=max_batch_cots,
+                       arena_d=arena.data_ptr() if arena is not None else None)
+    if arena is not None and arena.numel() * arena.element_size() < hbm_budget:
+        raise ValueError("arena smaller than hbm_budget")
     arr = (Instance * n)(*instances)
     t = (C.c_double * n)(*[float(a) for a in arrivals])
     lat = (C.c_double * n)()

@@ -52,7 +52,7 @@ class Instance(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_batch", I32), ("streams", I32), ("qc_chunk_bytes", C.c_size_t),
-                ("use_graph", I32), ("plain", I32), ("max_batch_cots", I64)]
+                ("use_graph", I32), ("plain", I32), ("max_batch_cots", I64), ("arena_d", P)]
 
 
 class ServerResult(C.Structure):

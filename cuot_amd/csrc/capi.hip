@@ -34,6 +34,12 @@ extern "C" int cuot_init(int device, cuot_ctx** out) {
   cuot_ctx* c = new cuot_ctx();
   c->device = device;
   c->stream = nullptr;  // legacy default stream unless the caller passes one
+  if (c->ensure_streams() != hipSuccess) {
+    c->release_all();
+    delete c;
+    cuot_set_last_error("stream creation failed", __FILE__, __LINE__);
+    return CUOT_ERR_HIP;
+  }
   *out = c;
   return CUOT_OK;
 }

@@ -123,6 +123,8 @@ struct cuot_server {
   cuot_ctx* ctx = nullptr;
   cuot_server_config cfg;
   char* arena_d = nullptr;
+  bool own_arena = true;  // false: the caller owns arena_d (cuot_server_config.arena_d)
+  bool pool_streams = false;  // the streams are the context's worker streams (not destroyed here)
   Arena arena;
   std::vector<hipStream_t> streams;
   std::vector<bool> busy;
@@ -415,17 +417,31 @@ extern "C" int cuot_server_new(cuot_ctx* ctx, const cuot_server_config* cfg, cuo
   s->ctx = ctx;
   s->cfg = *cfg;
   const int ns = cfg->plain ? 1 : (cfg->streams > 0 ? cfg->streams : 2);
-  if (hipSetDevice(ctx->device) != hipSuccess || hipMalloc((void**)&s->arena_d, cfg->hbm_budget) != hipSuccess) {
+  if (hipSetDevice(ctx->device) != hipSuccess) {
+    cuot_set_last_error("hipSetDevice failed", __FILE__, __LINE__);
+    delete s;
+    return CUOT_ERR_HIP;
+  }
+  s->own_arena = cfg->arena_d == nullptr;
+  if (!s->own_arena) {
+    s->arena_d = (char*)cfg->arena_d;
+  } else if (hipMalloc((void**)&s->arena_d, cfg->hbm_budget) != hipSuccess) {
     cuot_set_last_error("cannot allocate the HBM budget", __FILE__, __LINE__);
     delete s;
     return CUOT_ERR_NOMEM;
   }
   s->arena.init(cfg->hbm_budget);
+  // the context's worker streams when free (they hold distinct hardware
+  // queues); otherwise streams of the server's own
+  s->pool_streams = ns <= 2 && ctx->work[0] && !ctx->work_taken.exchange(true);
   for (int i = 0; i < ns; i++) {
-    hipStream_t st;
-    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
-      for (hipStream_t x : s->streams) hipStreamDestroy(x);
-      hipFree(s->arena_d);
+    hipStream_t st = s->pool_streams ? ctx->work[i] : nullptr;
+    if (!st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      if (!s->pool_streams)
+        for (hipStream_t x : s->streams) hipStreamDestroy(x);
+      else
+        ctx->work_taken = false;
+      if (s->own_arena) hipFree(s->arena_d);
       delete s;
       cuot_set_last_error("stream creation failed", __FILE__, __LINE__);
       return CUOT_ERR_HIP;
@@ -524,8 +540,11 @@ extern "C" int cuot_server_free(cuot_server* s) {
   }
   s->cv.notify_all();
   if (s->worker.joinable()) s->worker.join();
-  for (hipStream_t st : s->streams) hipStreamDestroy(st);
-  hipFree(s->arena_d);
+  if (s->pool_streams)
+    s->ctx->work_taken = false;
+  else
+    for (hipStream_t st : s->streams) hipStreamDestroy(st);
+  if (s->own_arena) hipFree(s->arena_d);
   delete s;
   return CUOT_OK;
 }
@@ -547,12 +566,14 @@ extern "C" int cuot_serve_replay(cuot_ctx* ctx, const cuot_server_config* cfg, c
   const bool check = relation_failures != nullptr;
   hipStream_t cst = nullptr;
   unsigned long long* fail_d = nullptr;
+  const bool own_cst = !s->pool_streams;  // the context's third worker stream checks pooled servers
   if (check) {
-    if (hipStreamCreateWithFlags(&cst, hipStreamNonBlocking) != hipSuccess ||
+    if (!own_cst) cst = ctx->work[2];
+    if ((own_cst && hipStreamCreateWithFlags(&cst, hipStreamNonBlocking) != hipSuccess) ||
         hipMalloc((void**)&fail_d, sizeof(unsigned long long)) != hipSuccess ||
         hipMemsetAsync(fail_d, 0, sizeof(unsigned long long), cst) != hipSuccess) {
       if (fail_d) hipFree(fail_d);
-      if (cst) hipStreamDestroy(cst);
+      if (cst && own_cst) hipStreamDestroy(cst);
       cuot_server_free(s);
       cuot_set_last_error("relation check set-up failed", __FILE__, __LINE__);
       return CUOT_ERR_HIP;
@@ -643,7 +664,7 @@ extern "C" int cuot_serve_replay(cuot_ctx* ctx, const cuot_server_config* cfg, c
       first_err = CUOT_ERR_HIP;
     *relation_failures = (int64_t)fails;
     hipFree(fail_d);
-    hipStreamDestroy(cst);
+    if (own_cst) hipStreamDestroy(cst);
   }
   if (batches) cuot_server_stats(s, batches, nullptr);
   cuot_server_free(s);

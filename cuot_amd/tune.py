@@ -11,7 +11,7 @@ speed.
 
 Stages and knobs:
   * "qc"  -- quasi-cyclic encode of one party at n COTs: FFT layers per upper
-             pass (qc_radix 2..5) and columns per tile (qc_cols 1, 2, 4, 8);
+             pass (qc_radix 4..7) and columns per tile (qc_cols 1, 2, 4, 8);
              features (log2 n, m, radix, log2 cols);
   * "ggm" -- GGM expansion of t trees of depth h: per-lane depth-first levels
              L (ggm_dfs, h-10 <= L <= min(h-8, 6)); features (h, log2 t, L).
@@ -142,8 +142,8 @@ def get_knobs() -> dict:
     return {"qc_radix": k.qc_radix, "qc_cols": k.qc_cols, "ggm_dfs": k.ggm_dfs, "lpn_wgs": k.lpn_wgs}
 
 
-QC_SPACE = [dict(qc_radix=r, qc_cols=c) for r, c in itertools.product((2, 3, 4, 5), (1, 2, 4, 8))]
-QC_DEFAULT = dict(qc_radix=5, qc_cols=8)
+QC_SPACE = [dict(qc_radix=r, qc_cols=c) for r, c in itertools.product((4, 5, 6, 7), (1, 2, 4, 8))]
+QC_DEFAULT = dict(qc_radix=7, qc_cols=2)
 
 
 def ggm_space(h: int):

@@ -350,6 +350,9 @@ typedef struct {
   int32_t use_graph;     /* replay batches through captured graphs */
   int32_t plain;         /* FIFO plain comparator */
   int64_t max_batch_cots; /* cap on the COTs of one batch (0 = 2^24); bounds batch duration */
+  void *arena_d;          /* optional caller-owned device arena of hbm_budget bytes (NULL: the server
+                             allocates and frees its own); lets a caller place the arena once, e.g. at
+                             start-up before device memory is fragmented, and reuse it across servers */
 } cuot_server_config;
 
 typedef struct {

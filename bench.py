@@ -349,14 +349,29 @@ def serve_leg(torch, cuot_amd, ctx, count, rate, budget_gb, sweep_rates=(), swee
     return res
 
 
+FERRET_CLIENTS = 8
+
+
 def serve_run(torch, cuot_amd, ctx, count, rate, budget_gb, arena=None):
     import numpy as np
 
     from cuot_amd.workloads import serving_arrivals
 
     arr = serving_arrivals(rate, count)
-    insts = [cuot_amd.make_instance_struct(wi.kind, wi.n, wi.t, wi.h, wi.k, wi.seed, k0=SOT_K0, k1=SOT_K1)
-             for _, wi in arr]
+    # Ferret requests come from FERRET_CLIENTS resident clients (round robin), each an
+    # iterating session of the emp-ot ferret_b13 size whose Delta and bootstrap stay in HBM
+    from cuot_amd.workloads import FERRET
+
+    insts, nf = [], 0
+    for _, wi in arr:
+        if wi.kind == FERRET:
+            nf += 1
+            insts.append(cuot_amd.make_instance_struct(FERRET, wi.n, 0, 0, 0, (0, 0),
+                                                       client=1 + nf % FERRET_CLIENTS))
+        else:
+            insts.append(cuot_amd.make_instance_struct(wi.kind, wi.n, wi.t, wi.h, wi.k, wi.seed, k0=SOT_K0,
+                                                       k1=SOT_K1))
+    fsess = cuot_amd.FerretConfig(n=10168320, k=452160, t=1270, h=13, ntasks=1, seed=cuot_amd.to_block(SEED))
     times = [a for a, _ in arr]
     total = sum(wi.n for _, wi in arr)
     budget = int(budget_gb * (1 << 30))
@@ -364,6 +379,8 @@ def serve_run(torch, cuot_amd, ctx, count, rate, budget_gb, arena=None):
     res = {"config": {"workload": "serving", "requests": count, "rate_hz": rate,
                       "offered_cots_s": round(total / span, 1) if span > 0 else None,
                       "sizes": "n = floor(2^(14 + 10u))", "kinds": "SOT / Ferret 50/50", "hbm_budget_bytes": budget,
+                      "ferret": f"{FERRET_CLIENTS} resident clients (iterating sessions n=10,168,320 k=452,160 "
+                                "t=1,270: Delta, bootstrap and the last extend's outputs in HBM between requests)",
                       "max_batch": 64, "max_batch_cots": 1 << 22, "streams": 2}}
     torch.cuda.empty_cache()  # the earlier legs' cached blocks go back to the device
     res["config"]["arena"] = "caller-owned, shared by the servers of the leg" if arena is not None else "server-owned"
@@ -373,7 +390,8 @@ def serve_run(torch, cuot_amd, ctx, count, rate, budget_gb, arena=None):
         # small ones behind it for a whole 2^24-COT encode
         lat, wall, nb, fails = cuot_amd.serve_replay(ctx, insts, times, hbm_budget=budget, plain=plain,
                                                      max_batch=64, streams=2, max_batch_cots=1 << 22,
-                                                     check_relation=True, arena=arena)
+                                                     check_relation=True, arena=arena,
+                                                     ferret_clients=FERRET_CLIENTS, ferret_session=fsess)
         res[name] = {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
                      "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
                      "mean_ms": round(float(lat.mean()) * 1e3, 3),

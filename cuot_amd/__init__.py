@@ -219,12 +219,14 @@ class Server:
     """The serving loop (cuot_server_*): submit requests, poll results, release."""
 
     def __init__(self, ctx: Context, hbm_budget: int, max_batch: int = 0, streams: int = 0, qc_chunk_bytes: int = 0,
-                 use_graph: bool = False, plain: bool = False, max_batch_cots: int = 0):
+                 use_graph: bool = False, plain: bool = False, max_batch_cots: int = 0, ferret_clients: int = 0,
+                 ferret_session: FerretConfig | None = None):
         self.ctx = ctx
         self.lib = ctx.lib
         self.cfg = ServerConfig(hbm_budget=hbm_budget, max_batch=max_batch, streams=streams,
                                 qc_chunk_bytes=qc_chunk_bytes, use_graph=int(use_graph), plain=int(plain),
-                                max_batch_cots=max_batch_cots)
+                                max_batch_cots=max_batch_cots, ferret_clients=ferret_clients,
+                                ferret_session=ferret_session or FerretConfig())
         h = C.c_void_p()
         check(self.lib.cuot_server_new(ctx.handle, C.byref(self.cfg), C.byref(h)), "cuot_server_new")
         self.handle = h
@@ -262,7 +264,8 @@ class Server:
 
 def serve_replay(ctx: Context, instances, arrivals, hbm_budget: int, plain: bool = False, max_batch: int = 0,
                  streams: int = 0, qc_chunk_bytes: int = 0, use_graph: bool = False, max_batch_cots: int = 0,
-                 check_relation: bool = False, arena=None):
+                 check_relation: bool = False, arena=None, ferret_clients: int = 0,
+                 ferret_session: FerretConfig | None = None):
     """Real-time replay of (arrival seconds, Instance) through a fresh server
     (arena: optional caller-owned device tensor of at least hbm_budget bytes).
 
@@ -273,7 +276,8 @@ def serve_replay(ctx: Context, instances, arrivals, hbm_budget: int, plain: bool
     n = len(instances)
     cfg = ServerConfig(hbm_budget=hbm_budget, max_batch=max_batch, streams=streams, qc_chunk_bytes=qc_chunk_bytes,
                        use_graph=int(use_graph), plain=int(plain), max_batch_cots=max_batch_cots,
-                       arena_d=arena.data_ptr() if arena is not None else None)
+                       arena_d=arena.data_ptr() if arena is not None else None, ferret_clients=ferret_clients,
+                       ferret_session=ferret_session or FerretConfig())
     if arena is not None and arena.numel() * arena.element_size() < hbm_budget:
         raise ValueError("arena smaller than hbm_budget")
     arr = (Instance * n)(*instances)
@@ -288,9 +292,10 @@ def serve_replay(ctx: Context, instances, arrivals, hbm_budget: int, plain: bool
 
 
 def make_instance_struct(kind: int, n: int, t: int, h: int, k: int, seed, ntasks: int = 1, k0=(3242342, 0),
-                         k1=(8993849, 0)) -> Instance:
-    """Instance record (SOT keys default to the SilentConfig leftKey/rightKey)."""
-    return Instance(kind=kind, h=h, n=n, t=t, k=k, ntasks=ntasks, reserved=0, k0=to_block(k0), k1=to_block(k1),
+                         k1=(8993849, 0), client: int = 0) -> Instance:
+    """Instance record (SOT keys default to the SilentConfig leftKey/rightKey;
+    client > 0: a Ferret request served from that resident client's session)."""
+    return Instance(kind=kind, h=h, n=n, t=t, k=k, ntasks=ntasks, client=client, k0=to_block(k0), k1=to_block(k1),
                     seed=to_block(seed))
 
 

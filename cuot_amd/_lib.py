@@ -46,13 +46,14 @@ class SilentConfig(C.Structure):
 
 
 class Instance(C.Structure):
-    _fields_ = [("kind", I32), ("h", I32), ("n", I64), ("t", I64), ("k", I64), ("ntasks", I32), ("reserved", I32),
+    _fields_ = [("kind", I32), ("h", I32), ("n", I64), ("t", I64), ("k", I64), ("ntasks", I32), ("client", I32),
                 ("k0", Block), ("k1", Block), ("seed", Block)]
 
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_batch", I32), ("streams", I32), ("qc_chunk_bytes", C.c_size_t),
-                ("use_graph", I32), ("plain", I32), ("max_batch_cots", I64), ("arena_d", P)]
+                ("use_graph", I32), ("plain", I32), ("max_batch_cots", I64), ("arena_d", P),
+                ("ferret_clients", I32), ("reserved2", I32), ("ferret_session", FerretConfig)]
 
 
 class ServerResult(C.Structure):

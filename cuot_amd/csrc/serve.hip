@@ -137,6 +137,18 @@ struct cuot_server {
   std::thread worker;
   bool stop = false;
   int64_t launched_batches = 0;
+  // resident Ferret clients (cuot_server_config.ferret_clients): an iterating
+  // session each, plus the outputs of its last extend (sender n_ext blocks,
+  // then receiver n_ext blocks) of which `left` usable COTs from `pos` on have
+  // not been handed out yet.  pos/left are touched by the scheduler thread
+  // only; busy is guarded by mu.
+  struct Client {
+    cuot_ferret_session* sess = nullptr;
+    char* buf = nullptr;
+    int64_t n_ext = 0, ot_limit = 0, pos = 0, left = 0;
+    bool busy = false;
+  };
+  std::vector<Client> clients;
 
   char* out0(const Req& r) const { return arena_d + r.out_off; }
   char* out1(const Req& r) const { return arena_d + r.out_off + a256((size_t)r.inst.n * 16); }
@@ -146,6 +158,11 @@ struct cuot_server {
   }
   void loop();
   int launch_batch(std::unique_lock<std::mutex>& lk, int s);
+  int launch_client(std::unique_lock<std::mutex>& lk, int s, int64_t ticket);
+  void erase_queued(int64_t ticket) {
+    auto it = std::find(queue.begin(), queue.end(), ticket);
+    if (it != queue.end()) queue.erase(it);
+  }
 };
 
 namespace {
@@ -156,6 +173,7 @@ struct Flight {
   std::vector<int64_t> tickets;
   size_t ws_off = 0, ws_bytes = 0;
   int stream = 0;
+  int client = 0;  // > 0: a resident-client request (the client is busy until the flight is reaped)
   std::atomic<bool> done{false};
   double t_done = 0;
   int status = CUOT_OK;
@@ -213,6 +231,23 @@ int plain_ws_bytes(const cuot_instance& in, size_t* ws) {
 // Called with the lock held; takes requests from the queue head.  Returns
 // 1 if a batch was launched, 0 if nothing could be admitted.
 int cuot_server::launch_batch(std::unique_lock<std::mutex>& lk, int s) {
+  // The first queued request that can start decides: a resident-client
+  // request whose client is idle runs alone on the client's session; a
+  // request without client starts a batch of the queued requests without
+  // client (in arrival order).  Requests of a busy client wait their turn.
+  for (int64_t t : queue) {
+    Req& r = reqs[t];
+    if (r.inst.client == 0) break;
+    if (r.inst.client < 0 || r.inst.client > (int)clients.size() || r.inst.kind != CUOT_MODE_FERRET) {
+      r.state = 2;
+      r.status = CUOT_ERR_ARG;
+      r.t_done = now_s();
+      erase_queued(t);
+      cv.notify_all();
+      return 1;
+    }
+    if (!clients[r.inst.client - 1].busy) return launch_client(lk, s, t);
+  }
   std::vector<int64_t> take;
   std::vector<cuot_instance> insts;
   cuot_batch* b = nullptr;
@@ -223,6 +258,7 @@ int cuot_server::launch_batch(std::unique_lock<std::mutex>& lk, int s) {
   for (int64_t t : queue) {
     if ((int)take.size() >= maxb) break;
     Req& r = reqs[t];
+    if (r.inst.client != 0) continue;  // served by its client's session
     if (!take.empty() && cots + r.inst.n > max_cots) break;  // keep batches short
     std::vector<cuot_instance> trial = insts;
     trial.push_back(r.inst);
@@ -234,7 +270,7 @@ int cuot_server::launch_batch(std::unique_lock<std::mutex>& lk, int s) {
         r.state = 2;
         r.status = rc;
         r.t_done = now_s();
-        queue.pop_front();
+        erase_queued(t);
         cv.notify_all();
         return 1;
       }
@@ -256,7 +292,7 @@ int cuot_server::launch_batch(std::unique_lock<std::mutex>& lk, int s) {
         r.state = 2;
         r.status = CUOT_ERR_NOMEM;
         r.t_done = now_s();
-        queue.pop_front();
+        erase_queued(t);
         cv.notify_all();
         return 1;
       }
@@ -283,8 +319,8 @@ int cuot_server::launch_batch(std::unique_lock<std::mutex>& lk, int s) {
     r.out_bytes = out_bytes_of(r.inst);
     r.state = 1;
     r.t_start = now_s();
-    queue.pop_front();
   }
+  for (int64_t t : take) erase_queued(t);
   arena.alloc(ws, &f->ws_off);
   busy[s] = true;
   inflight.push_back(f);
@@ -346,6 +382,79 @@ int cuot_server::launch_batch(std::unique_lock<std::mutex>& lk, int s) {
   return 1;
 }
 
+// A resident-client request (called with the lock held): its outputs are
+// taken from the arena, then on stream s the client's unused COTs are copied
+// and further extends run until n COTs are handed out (FerretCOT::rcot,
+// ferret_cot.hpp:171-209).  Returns 1 if launched (or failed), 0 if the arena
+// has no room yet.
+int cuot_server::launch_client(std::unique_lock<std::mutex>& lk, int s, int64_t t) {
+  Req& r = reqs[t];
+  Client& cl = clients[r.inst.client - 1];
+  const size_t ob = out_bytes_of(r.inst);
+  size_t off = 0;
+  if (!arena.alloc(ob, &off)) {
+    if (inflight.empty() && ob > arena.size) {
+      r.state = 2;
+      r.status = CUOT_ERR_NOMEM;
+      r.t_done = now_s();
+      erase_queued(t);
+      cv.notify_all();
+      return 1;
+    }
+    return 0;
+  }
+  r.out_off = off;
+  r.out_bytes = ob;
+  r.state = 1;
+  r.t_start = now_s();
+  erase_queued(t);
+  Flight* f = new Flight();
+  f->srv = this;
+  f->tickets.push_back(t);
+  f->stream = s;
+  f->client = r.inst.client;
+  cl.busy = true;
+  busy[s] = true;
+  inflight.push_back(f);
+  launched_batches++;
+  const int64_t n = r.inst.n;
+  char* o0 = out0(r);
+  char* o1 = out1(r);
+  char* dl = delta(r);
+  lk.unlock();
+  hipStream_t st = streams[s];
+  int rc = CUOT_OK;
+  int64_t done = 0;
+  while (!rc && done < n) {
+    if (cl.left == 0) {
+      rc = cuot_ferret_session_extend(cl.sess, (cuot_block*)cl.buf, (cuot_block*)(cl.buf + cl.n_ext * 16), st);
+      cl.pos = 0;
+      cl.left = rc ? 0 : cl.ot_limit;
+      if (rc) break;
+    }
+    const int64_t take = std::min(cl.left, n - done);
+    if (hipMemcpyAsync(o0 + done * 16, cl.buf + cl.pos * 16, (size_t)take * 16, hipMemcpyDeviceToDevice, st) ||
+        hipMemcpyAsync(o1 + done * 16, cl.buf + (cl.n_ext + cl.pos) * 16, (size_t)take * 16,
+                       hipMemcpyDeviceToDevice, st))
+      rc = CUOT_ERR_HIP;
+    cl.pos += take;
+    cl.left -= take;
+    done += take;
+  }
+  if (!rc) rc = cuot_ferret_session_delta(cl.sess, (cuot_block*)dl, st);
+  if (!rc && hipLaunchHostFunc(st, on_done, f) != hipSuccess) rc = CUOT_ERR_HIP;
+  lk.lock();
+  if (rc) {
+    lk.unlock();
+    hipStreamSynchronize(st);
+    lk.lock();
+    f->status = rc;
+    f->t_done = now_s();
+    f->done.store(true);
+  }
+  return 1;
+}
+
 void cuot_server::loop() {
   hipSetDevice(ctx->device);
   std::unique_lock<std::mutex> lk(mu);
@@ -367,7 +476,8 @@ void cuot_server::loop() {
         rit->second.status = f->status;
         rit->second.t_done = f->t_done;
       }
-      arena.release(f->ws_off, f->ws_bytes);
+      if (f->ws_bytes) arena.release(f->ws_off, f->ws_bytes);
+      if (f->client > 0) clients[f->client - 1].busy = false;
       busy[f->stream] = false;
       finished.push_back(f);
       it = inflight.erase(it);
@@ -412,6 +522,7 @@ extern "C" int cuot_server_new(cuot_ctx* ctx, const cuot_server_config* cfg, cuo
   CUOT_CHECK_ARG(ctx && cfg && out, "NULL argument");
   CUOT_CHECK_ARG(cfg->hbm_budget >= (1u << 20), "hbm_budget below 1 MiB");
   CUOT_CHECK_ARG(cfg->streams >= 0 && cfg->streams <= 8, "streams must be in [0, 8]");
+  CUOT_CHECK_ARG(cfg->ferret_clients >= 0 && cfg->ferret_clients <= 64, "ferret_clients must be in [0, 64]");
   *out = nullptr;
   cuot_server* s = new cuot_server();
   s->ctx = ctx;
@@ -422,15 +533,56 @@ extern "C" int cuot_server_new(cuot_ctx* ctx, const cuot_server_config* cfg, cuo
     delete s;
     return CUOT_ERR_HIP;
   }
+  auto free_clients = [s] {
+    for (auto& cl : s->clients) {
+      if (cl.sess) cuot_ferret_session_free(cl.sess);
+      if (cl.buf) hipFree(cl.buf);
+    }
+    s->clients.clear();
+  };
+  // resident Ferret clients: their device memory counts against the budget
+  size_t sess_bytes = 0;
+  if (cfg->ferret_clients > 0) {
+    size_t f0 = 0, f1 = 0, tot = 0;
+    hipMemGetInfo(&f0, &tot);
+    for (int c = 1; c <= cfg->ferret_clients; c++) {
+      cuot_ferret_config fc = cfg->ferret_session;
+      fc.seed.lo ^= (uint64_t)c;
+      cuot_server::Client cl;
+      int rc = cuot_ferret_session_new(ctx, &fc, &cl.sess);
+      if (!rc) rc = cuot_ferret_session_info(cl.sess, nullptr, &cl.ot_limit, nullptr);
+      cl.n_ext = fc.n;
+      if (!rc && hipMalloc((void**)&cl.buf, (size_t)fc.n * 32) != hipSuccess) {
+        cuot_set_last_error("cannot allocate a client's extend buffer", __FILE__, __LINE__);
+        rc = CUOT_ERR_NOMEM;
+      }
+      s->clients.push_back(cl);
+      if (rc) {
+        free_clients();
+        delete s;
+        return rc;
+      }
+    }
+    hipMemGetInfo(&f1, &tot);
+    sess_bytes = f0 > f1 ? f0 - f1 : 0;
+    if (sess_bytes + (1u << 20) > cfg->hbm_budget) {
+      free_clients();
+      delete s;
+      cuot_set_last_error("hbm_budget does not hold the resident Ferret clients", __FILE__, __LINE__);
+      return CUOT_ERR_NOMEM;
+    }
+  }
+  const size_t arena_bytes = cfg->hbm_budget - sess_bytes;
   s->own_arena = cfg->arena_d == nullptr;
   if (!s->own_arena) {
     s->arena_d = (char*)cfg->arena_d;
-  } else if (hipMalloc((void**)&s->arena_d, cfg->hbm_budget) != hipSuccess) {
+  } else if (hipMalloc((void**)&s->arena_d, arena_bytes) != hipSuccess) {
+    free_clients();
     cuot_set_last_error("cannot allocate the HBM budget", __FILE__, __LINE__);
     delete s;
     return CUOT_ERR_NOMEM;
   }
-  s->arena.init(cfg->hbm_budget);
+  s->arena.init(arena_bytes);
   // the context's worker streams when free (they hold distinct hardware
   // queues); otherwise streams of the server's own
   s->pool_streams = ns <= 2 && ctx->work[0] && !ctx->work_taken.exchange(true);
@@ -442,6 +594,7 @@ extern "C" int cuot_server_new(cuot_ctx* ctx, const cuot_server_config* cfg, cuo
       else
         ctx->work_taken = false;
       if (s->own_arena) hipFree(s->arena_d);
+      free_clients();
       delete s;
       cuot_set_last_error("stream creation failed", __FILE__, __LINE__);
       return CUOT_ERR_HIP;
@@ -459,6 +612,9 @@ extern "C" int cuot_server_submit(cuot_server* s, const cuot_instance* inst, int
   CUOT_CHECK_ARG(s && inst && ticket, "NULL argument");
   CUOT_CHECK_ARG(inst->kind == CUOT_MODE_SOT || inst->kind == CUOT_MODE_FERRET, "bad instance kind");
   CUOT_CHECK_ARG(inst->n >= 1, "n < 1");
+  CUOT_CHECK_ARG(inst->client == 0 || (inst->kind == CUOT_MODE_FERRET && inst->client > 0 &&
+                                       inst->client <= (int)s->clients.size()),
+                 "client must be 0 or a resident Ferret client (Ferret requests only)");
   {
     std::lock_guard<std::mutex> g(s->mu);
     if (s->stop) {
@@ -545,6 +701,10 @@ extern "C" int cuot_server_free(cuot_server* s) {
   else
     for (hipStream_t st : s->streams) hipStreamDestroy(st);
   if (s->own_arena) hipFree(s->arena_d);
+  for (auto& cl : s->clients) {
+    cuot_ferret_session_free(cl.sess);
+    hipFree(cl.buf);
+  }
   delete s;
   return CUOT_OK;
 }

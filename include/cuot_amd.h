@@ -300,7 +300,8 @@ typedef struct {
   int64_t t;      /* trees */
   int64_t k;      /* Ferret bootstrap COTs (0 for SOT) */
   int32_t ntasks; /* Ferret LPN task partition (emp-ot threads) */
-  int32_t reserved;
+  int32_t client; /* serving only: 0, or a resident Ferret client in [1, ferret_clients] whose
+                     iterating session serves this Ferret request (n COTs; h, t, k, seed unused) */
   cuot_block k0, k1; /* SOT GGM keys (ignored for Ferret: TwoKeyPRP keys) */
   cuot_block seed;
 } cuot_instance;
@@ -351,8 +352,17 @@ typedef struct {
   int32_t plain;         /* FIFO plain comparator */
   int64_t max_batch_cots; /* cap on the COTs of one batch (0 = 2^24); bounds batch duration */
   void *arena_d;          /* optional caller-owned device arena of hbm_budget bytes (NULL: the server
-                             allocates and frees its own); lets a caller place the arena once, e.g. at
-                             start-up before device memory is fragmented, and reuse it across servers */
+                             allocates and frees its own), e.g. one arena reused by several servers */
+  int32_t ferret_clients; /* resident Ferret clients (0 = none): client c holds an iterating Ferret session
+                             (Delta, bootstrap COTs and its last extend's outputs stay in HBM between
+                             requests; cuot_ferret_session_*) with the parameters of ferret_session and
+                             seed ferret_session.seed with lo ^= c.  A Ferret request of client c takes
+                             its n COTs as FerretCOT::rcot does (ferret_cot.hpp:171-209): the unused
+                             outputs of the client's last extend first, then new extends; requests of
+                             one client run in arrival order.  The sessions' device memory counts
+                             against hbm_budget. */
+  int32_t reserved2;
+  cuot_ferret_config ferret_session;
 } cuot_server_config;
 
 typedef struct {

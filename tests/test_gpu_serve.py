@@ -96,3 +96,53 @@ def test_replay_completes(ctx):
     assert np.all(np.isfinite(lat)) and np.all(lat > 0)
     assert wall >= arrivals[-1]
     assert fails == 0, f"{fails} indices of the served results break the COT relation"
+
+
+@pytest.mark.parametrize("plain", [False, True])
+def test_resident_ferret_clients_match_rcot(ctx, plain):
+    """Ferret requests of resident clients are served from each client's
+    iterating session (Delta and bootstrap stay in HBM between requests): the
+    outputs of a client's requests, in arrival order, equal FerretCOT.rcot
+    calls of the same sizes on a session with the client's seed, whatever SOT
+    requests run in between; Delta is the session's and the COT relation holds."""
+    import torch
+
+    import cuot_amd
+    from cuot_amd import FerretConfig, make_instance_struct
+    from cuot_amd.ferret import FerretCOT
+    from cuot_amd.workloads import FERRET, SOT, make_instance
+
+    print(f"resident ferret clients plain={plain}", flush=True)
+    P = dict(n=40000, k=2000, t=5, h=13)
+    seed = (0x1234, 0x5678)
+    fcfg = FerretConfig(ntasks=1, seed=cuot_amd.to_block(seed), **P)
+    srv = cuot_amd.Server(ctx, hbm_budget=2 << 30, plain=plain, ferret_clients=2, ferret_session=fcfg)
+    sot = make_instance(SOT, 5000, (9, 9))
+    reqs = [(1, 1000), (2, 50000), (0, None), (1, 40000), (2, 10), (1, 37807), (0, None), (2, 3)]
+    tickets = []
+    for client, n in reqs:
+        if client == 0:
+            inst = make_instance_struct(SOT, sot.n, sot.t, sot.h, 0, (9, 9), k0=SOT_K0, k1=SOT_K1)
+        else:
+            inst = make_instance_struct(FERRET, n, 0, 0, 0, (0, 0), client=client)
+        tickets.append(srv.submit(inst))
+    res = [srv.poll(t, wait=True) for t in tickets]
+    assert all(r.status == 0 for r in res)
+    refs = {c: FerretCOT(ctx, seed=(seed[0] ^ c, seed[1]), **P) for c in (1, 2)}
+    for (client, n), r in zip(reqs, res):
+        if client == 0:
+            continue
+        s_ref, r_ref = refs[client].rcot(n)
+        torch.cuda.synchronize()
+        s = _dev_to_np(r.out0, n)
+        rr = _dev_to_np(r.out1, n)
+        d = _dev_to_np(r.delta, 1)[0]
+        assert np.array_equal(s, to_np(s_ref)) and np.array_equal(rr, to_np(r_ref))
+        assert np.array_equal(d, to_np(refs[client].Delta)[0])
+        lsb = (rr[:, 0] & 1).astype(bool)
+        assert np.array_equal(rr, s ^ np.where(lsb[:, None], d, 0).astype(np.uint64))
+    for t in tickets:
+        srv.release(t)
+    for f in refs.values():
+        f.close()
+    srv.close()

@@ -45,6 +45,7 @@ struct PrepGroup {
 struct QcChunk {
   int m = 0;
   std::vector<std::pair<int, int>> parties;  // (instance, 0 sender / 1 receiver)
+  int64_t nmax = 0;                          // largest n of the chunk
   size_t io_off = 0, ahat_off = 0;
   bool aux = false;  // second stream under CUOT_BATCH_OVERLAP (own scratch region)
 };
@@ -168,8 +169,10 @@ int plan_batch(cuot_batch* b) {
         if (!cur.parties.empty() && !fits) {
           b->qc.push_back(cur);
           cur.parties.clear();
+          cur.nmax = 0;
         }
         cur.parties.emplace_back(i, party);
+        cur.nmax = std::max(cur.nmax, b->inst[i].n);
       }
     }
     if (!cur.parties.empty()) b->qc.push_back(cur);
@@ -399,7 +402,7 @@ hipError_t launch_all(cuot_batch* b, uint32_t stages, hipStream_t st, hipStream_
       const bool on_ax = two && c.aux;
       char* scr = ws + (c.aux ? b->qc_scratch2_off : b->qc_scratch_off);
       if ((e = cuot_launch_qc_encode_batch((const QcIo*)(ws + c.io_off), (const uint32_t* const*)(ws + c.ahat_off),
-                                           (int)c.parties.size(), c.m, scr, on_ax ? ax : st)))
+                                           (int)c.parties.size(), c.m, c.nmax, scr, on_ax ? ax : st)))
         return e;
     }
   if (two) {

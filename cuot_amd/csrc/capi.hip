@@ -191,8 +191,8 @@ extern "C" int cuot_lpn_dual(cuot_ctx* ctx, const cuot_lpn_params* p, const cuot
 }
 
 extern "C" int64_t cuot_qc_ahat_len(int64_t n) {
-  if (n < 1) return 0;
-  return (int64_t)1 << cuot_qc_log_len(n);
+  if (n < 1 || cuot_qc_log_len(n) > 30) return 0;
+  return cuot_qc_ahat_words(n);
 }
 
 extern "C" size_t cuot_qc_scratch_bytes(int64_t n) {
@@ -263,7 +263,7 @@ extern "C" int cuot_bit_untranspose(cuot_ctx* ctx, const uint8_t* rows_d, int64_
 
 // ---- execution knobs ---------------------------------------------------------
 namespace {
-CuotKnobs g_knobs = {0, 0, 0, 0};
+CuotKnobs g_knobs = {0, 0, 0, 0, 0};
 }
 
 const CuotKnobs& cuot_knobs() { return g_knobs; }
@@ -275,10 +275,12 @@ extern "C" int cuot_set_knobs(const cuot_knob_values* k) {
                  "qc_cols must be 0, 1, 2, 4 or 8");
   CUOT_CHECK_ARG(k->ggm_dfs >= 0 && k->ggm_dfs <= 6, "ggm_dfs must be in [0, 6]");
   CUOT_CHECK_ARG(k->lpn_wgs >= 0, "lpn_wgs must be >= 0");
+  CUOT_CHECK_ARG(k->qc_mfma_max >= -1, "qc_mfma_max must be >= -1");
   g_knobs.qc_radix = k->qc_radix;
   g_knobs.qc_cols = k->qc_cols;
   g_knobs.ggm_dfs = k->ggm_dfs;
   g_knobs.lpn_wgs = k->lpn_wgs;
+  g_knobs.qc_mfma_max = k->qc_mfma_max;
   return CUOT_OK;
 }
 
@@ -288,6 +290,7 @@ extern "C" int cuot_get_knobs(cuot_knob_values* k) {
   k->qc_cols = g_knobs.qc_cols;
   k->ggm_dfs = g_knobs.ggm_dfs;
   k->lpn_wgs = g_knobs.lpn_wgs;
+  k->qc_mfma_max = g_knobs.qc_mfma_max;
   return CUOT_OK;
 }
 

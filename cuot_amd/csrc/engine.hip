@@ -53,7 +53,7 @@ size_t SotPlan::carve(void* ws) {
   M1 = c.take<blk>((h + 1) * t);
   csum = c.take<blk>(t * h);
   fin = c.take<blk>(t);
-  ahat = c.take<uint32_t>((size_t)cuot_qc_log_len(n) < 63 ? ((size_t)1 << cuot_qc_log_len(n)) : 0);
+  ahat = c.take<uint32_t>(cuot_qc_log_len(n) <= 30 ? (size_t)cuot_qc_ahat_words(n) : 0);
   leaves_s = c.take<blk>(2 * n);
   leaves_r = c.take<blk>(2 * n);
   qc_scratch = shared_qc ? nullptr : c.take<char>(cuot_qc_scratch_bytes_for(n));

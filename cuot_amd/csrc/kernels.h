@@ -76,15 +76,25 @@ struct QcIo {
 };
 struct QcPrep {
   const uint8_t* a_bits;  // n bits of the circulant's first column
-  uint32_t* ahat;         // 2^m words
+  uint32_t* ahat;         // cuot_qc_ahat_words(n) words
   int64_t n;
 };
 size_t cuot_qc_batch_scratch_bytes(int m, int B);
 bool cuot_qc_batchable(int m);   // B > 1 allowed for transform length 2^m
 hipError_t cuot_qc_configure();  // one-time device set-up (outside graph capture)
 hipError_t cuot_launch_qc_prepare_batch(const QcPrep* prep_d, int B, int m, hipStream_t st);
-hipError_t cuot_launch_qc_encode_batch(const QcIo* io_d, const uint32_t* const* ahats_d, int B, int m,
+// nmax: the largest n of the batch (host side)
+hipError_t cuot_launch_qc_encode_batch(const QcIo* io_d, const uint32_t* const* ahats_d, int B, int m, int64_t nmax,
                                        void* scratch, hipStream_t st);
+// Small codes on the matrix cores (qc_mfma.hip).  For 17 <= n <= 2^16 the
+// prepared words are the 2^m transform words followed by a's n bits, so either
+// encoder can run on them; the encode picks by cuot_qc_mfma_selected.
+bool cuot_qc_mfma_eligible(int64_t n);
+bool cuot_qc_mfma_selected(int64_t n);  // eligible and at or below the crossover knob
+int64_t cuot_qc_ahat_words(int64_t n);
+hipError_t cuot_launch_qc_keep_bits(const QcPrep* pr, QcPrep pr1, int B, int m, int64_t nmax, hipStream_t st);
+hipError_t cuot_launch_qc_mfma(const QcIo* io, QcIo io1, const uint32_t* bits1, const uint32_t* const* ahats, int B,
+                               int m, int64_t nmax, void* scratch, hipStream_t st);
 
 // Sparse encode and bit transpose (bits.hip).
 hipError_t cuot_launch_sparse(const uint32_t* a_words, int64_t n, const int64_t* pos, int w, uint32_t* out,
@@ -121,6 +131,7 @@ struct CuotKnobs {
   int qc_cols;   // FFT columns per tile (1, 2, 4, 8)
   int ggm_dfs;   // per-lane depth-first levels L (h - 10 <= L <= min(h - 8, 6))
   int lpn_wgs;   // LPN grid cap
+  int qc_mfma_max;  // matrix-core QC encoder for n at or below (-1: never)
 };
 const CuotKnobs& cuot_knobs();
 

@@ -1505,13 +1505,21 @@ hipError_t cuot_launch_qc_prepare(const uint8_t* a_bits, int64_t n, uint32_t* ah
   p1.a_bits = a_bits;
   p1.ahat = ahat;
   p1.n = n;
-  return prepare_batch(nullptr, p1, 1, cuot_qc_log_len(n), st);
+  const int m = cuot_qc_log_len(n);
+  hipError_t e = prepare_batch(nullptr, p1, 1, m, st);
+  if (e == hipSuccess && cuot_qc_mfma_eligible(n)) e = cuot_launch_qc_keep_bits(nullptr, p1, 1, m, n, st);
+  return e;
 }
 
 hipError_t cuot_launch_qc_prepare_batch(const QcPrep* prep_d, int B, int m, hipStream_t st) {
   QcPrep p1;
   memset(&p1, 0, sizeof(p1));
-  return prepare_batch(prep_d, p1, B, m, st);
+  hipError_t e = prepare_batch(prep_d, p1, B, m, st);
+  // members small enough for the matrix-core encoder also keep a's bits
+  const int64_t nlo = ((int64_t)1 << (m + 2)) + 1, nhi = (int64_t)1 << (m + 3);
+  if (e == hipSuccess && cuot_qc_mfma_eligible(nlo))
+    e = cuot_launch_qc_keep_bits(prep_d, p1, B, m, cuot_qc_mfma_eligible(nhi) ? nhi : nlo, st);
+  return e;
 }
 
 hipError_t cuot_launch_qc_encode(const uint32_t* ahat, int64_t n, const void* x, void* y, void* scratch,
@@ -1520,12 +1528,17 @@ hipError_t cuot_launch_qc_encode(const uint32_t* ahat, int64_t n, const void* x,
   io1.x = (const uint4*)x;
   io1.y = (uint4*)y;
   io1.n = n;
-  return encode_batch(nullptr, io1, ahat, nullptr, 1, cuot_qc_log_len(n), scratch, st);
+  const int m = cuot_qc_log_len(n);
+  if (cuot_qc_mfma_selected(n))
+    return cuot_launch_qc_mfma(nullptr, io1, ahat + ((int64_t)1 << m), nullptr, 1, m, n, scratch, st);
+  return encode_batch(nullptr, io1, ahat, nullptr, 1, m, scratch, st);
 }
 
-hipError_t cuot_launch_qc_encode_batch(const QcIo* io_d, const uint32_t* const* ahats_d, int B, int m, void* scratch,
-                                       hipStream_t st) {
+hipError_t cuot_launch_qc_encode_batch(const QcIo* io_d, const uint32_t* const* ahats_d, int B, int m, int64_t nmax,
+                                       void* scratch, hipStream_t st) {
   QcIo io1;
   memset(&io1, 0, sizeof(io1));
+  if (m >= 2 && cuot_qc_mfma_selected(nmax))
+    return cuot_launch_qc_mfma(io_d, io1, nullptr, ahats_d, B, m, nmax, scratch, st);
   return encode_batch(io_d, io1, nullptr, ahats_d, B, m, scratch, st);
 }

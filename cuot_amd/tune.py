@@ -131,15 +131,16 @@ class CostModel:
 
 
 # ---- knobs -----------------------------------------------------------------------
-def set_knobs(qc_radix=0, qc_cols=0, ggm_dfs=0, lpn_wgs=0):
-    k = KnobValues(qc_radix, qc_cols, ggm_dfs, lpn_wgs)
+def set_knobs(qc_radix=0, qc_cols=0, ggm_dfs=0, lpn_wgs=0, qc_mfma_max=0):
+    k = KnobValues(qc_radix, qc_cols, ggm_dfs, lpn_wgs, qc_mfma_max)
     check(_lib.load().cuot_set_knobs(C.byref(k)), "cuot_set_knobs")
 
 
 def get_knobs() -> dict:
     k = KnobValues()
     check(_lib.load().cuot_get_knobs(C.byref(k)), "cuot_get_knobs")
-    return {"qc_radix": k.qc_radix, "qc_cols": k.qc_cols, "ggm_dfs": k.ggm_dfs, "lpn_wgs": k.lpn_wgs}
+    return {"qc_radix": k.qc_radix, "qc_cols": k.qc_cols, "ggm_dfs": k.ggm_dfs, "lpn_wgs": k.lpn_wgs,
+            "qc_mfma_max": k.qc_mfma_max}
 
 
 QC_SPACE = [dict(qc_radix=r, qc_cols=c) for r, c in itertools.product((4, 5, 6, 7), (1, 2, 4, 8))]
@@ -153,7 +154,12 @@ def ggm_space(h: int):
 
 
 def qc_log_len(n: int) -> int:
-    return int(_lib.load().cuot_qc_ahat_len(n)).bit_length() - 1
+    """Transform length 2^m of the FFT encoder: the least m >= 1 with 2^m >= 2 ceil(n/16) - 1."""
+    nc = (n + 15) // 16
+    m = 1
+    while (1 << m) < 2 * nc - 1:
+        m += 1
+    return m
 
 
 def qc_features(n: int, knobs: dict):

@@ -112,10 +112,13 @@ int cuot_lpn_dual(cuot_ctx *ctx, const cuot_lpn_params *p, const cuot_block *noi
  * bit-rows of the blocks at once: the workload's 2n -> n circulant code.
  * Replaces QuasiCyclic::encode_dense's float32 cuFFT path
  * (silent/quasi_cyclic.cu:118-128) and its per-8-row batching by an exact
- * additive FFT over GF(2^32).  a_bits_d: n bits.  ahat_d: cuot_qc_ahat_len(n)
- * words computed once per code by cuot_qc_prepare (QuasiCyclic::QuasiCyclic,
- * silent/quasi_cyclic.cu:51-106).  scratch_d: cuot_qc_scratch_bytes(n) bytes.
- * y_d may alias x_d. */
+ * additive FFT over GF(2^32), or, for small codes (17 <= n <= 2^16, at or
+ * below the knob qc_mfma_max), by the dense 0/1 int8 product on the matrix
+ * cores with int32 sums taken mod 2.  a_bits_d: n bits.  ahat_d:
+ * cuot_qc_ahat_len(n) words computed once per code by cuot_qc_prepare
+ * (QuasiCyclic::QuasiCyclic, silent/quasi_cyclic.cu:51-106): the transform,
+ * followed for small codes by a's bits, so both encoders run on it.
+ * scratch_d: cuot_qc_scratch_bytes(n) bytes.  y_d may alias x_d. */
 int64_t cuot_qc_ahat_len(int64_t n);
 size_t cuot_qc_scratch_bytes(int64_t n);
 int cuot_qc_prepare(cuot_ctx *ctx, int64_t n, const uint8_t *a_bits_d, uint32_t *ahat_d, void *scratch_d,
@@ -421,6 +424,8 @@ typedef struct {
   int32_t qc_cols;  /* FFT columns per tile: 1, 2, 4 or 8 */
   int32_t ggm_dfs;  /* GGM per-lane depth-first levels (valid: h-10 <= L <= min(h-8, 6); else the rule) */
   int32_t lpn_wgs;  /* LPN grid cap */
+  int32_t qc_mfma_max; /* QC encodes with 17 <= n <= this run as a 0/1 int8 matrix-core product
+                          (at most 2^16; -1: never, 0: the built-in crossover) */
 } cuot_knob_values;
 int cuot_set_knobs(const cuot_knob_values *k);
 int cuot_get_knobs(cuot_knob_values *k);

@@ -1,5 +1,6 @@
-"""GPU parity: exact QC encoder (additive FFT), sparse encoder and the bit
-transpose against the CPU oracle, bit-exact."""
+"""GPU parity: exact QC encoder (additive FFT and, for small codes, the int8
+matrix-core product), sparse encoder and the bit transpose against the CPU
+oracle, bit-exact."""
 import numpy as np
 import pytest
 
@@ -65,6 +66,86 @@ def test_qc_special_cases(ctx):
     xs = np.zeros(n // 8, dtype=np.uint8)
     xs[s >> 3] |= 1 << (s & 7)
     assert np.array_equal(qc_gpu(ctx, x, xs, n), x[:n] ^ x[n + (np.arange(n) - s) % n])
+
+
+@pytest.fixture
+def knob_mfma():
+    """Sets the matrix-core crossover for one test and restores the rule after."""
+    from cuot_amd import tune
+
+    def set_(v):
+        tune.set_knobs(qc_mfma_max=v)
+
+    yield set_
+    tune.set_knobs()
+
+
+# small codes on the int8 matrix cores: one slice (n <= 256, direct write) and
+# several (atomic XOR), n around the 32-row and 16-k tile edges
+@pytest.mark.parametrize("n", [17, 31, 32, 33, 255, 256, 257, 1000, 4099, 10007])
+def test_qc_mfma_vs_brute(ctx, knob_mfma, n):
+    print(f"qc mfma brute n={n}", flush=True)
+    knob_mfma(1 << 16)
+    r = rng(n + 11)
+    a, x = rand_bits(r, n), rand_blocks(r, 2 * n)
+    assert np.array_equal(qc_gpu(ctx, x, a, n), po.qc_encode_brute(x, a, n))
+
+
+# the two encoders on one prepared code (prepare keeps both forms), against
+# the oracle's Karatsuba product, up to the largest matrix-core size
+@pytest.mark.parametrize("n", [12345, 1 << 15, 40000, 1 << 16])
+def test_qc_mfma_and_fft_agree(ctx, knob_mfma, n):
+    import torch
+
+    print(f"qc mfma/fft n={n}", flush=True)
+    r = rng(n + 13)
+    a, x = rand_bits(r, n), rand_blocks(r, 2 * n)
+    want = po.qc_encode_fast(x, a, n, nthreads=8)
+    ahat = torch.zeros(ctx.qc_ahat_len(n), dtype=torch.int32, device="cuda")
+    scratch = torch.zeros(ctx.qc_scratch_bytes(n), dtype=torch.uint8, device="cuda")
+    ctx.qc_prepare(n, dev_bits(a, n), ahat, scratch)
+    xd = to_dev(x)
+    for lim in (1 << 16, -1):
+        knob_mfma(lim)
+        y = dev_blocks(n)
+        ctx.qc_encode(n, ahat, xd, y, scratch)
+        assert np.array_equal(to_np(y)[:n], want), lim
+
+
+# y over the first half of x (the C ABI allows the alias), both output modes
+@pytest.mark.parametrize("n", [200, 5000])
+def test_qc_mfma_in_place(ctx, knob_mfma, n):
+    import torch
+
+    knob_mfma(1 << 16)
+    r = rng(n + 17)
+    a, x = rand_bits(r, n), rand_blocks(r, 2 * n)
+    ahat = torch.zeros(ctx.qc_ahat_len(n), dtype=torch.int32, device="cuda")
+    scratch = torch.zeros(ctx.qc_scratch_bytes(n), dtype=torch.uint8, device="cuda")
+    ctx.qc_prepare(n, dev_bits(a, n), ahat, scratch)
+    xd = to_dev(x)
+    ctx.qc_encode(n, ahat, xd, xd, scratch)
+    assert np.array_equal(to_np(xd)[:n], po.qc_encode_brute(x, a, n))
+
+
+def test_qc_mfma_special_cases(ctx, knob_mfma):
+    knob_mfma(1 << 16)
+    n = 3000
+    r = rng(5)
+    x = rand_blocks(r, 2 * n)
+    one = np.zeros((n + 7) // 8, dtype=np.uint8)
+    one[0] = 1
+    assert np.array_equal(qc_gpu(ctx, x, one, n), x[:n] ^ x[n:])
+    s = 2999
+    xs = np.zeros((n + 7) // 8, dtype=np.uint8)
+    xs[s >> 3] |= 1 << (s & 7)
+    assert np.array_equal(qc_gpu(ctx, x, xs, n), x[:n] ^ x[n + (np.arange(n) - s) % n])
+    ones = rand_bits(r, n)
+    ones[:] = 0xFF
+    ones[-1] &= (1 << (n % 8)) - 1 if n % 8 else 0xFF
+    # all ones: y_i = x_i ^ XOR of the whole second half
+    tot = np.bitwise_xor.reduce(x[n:], axis=0)
+    assert np.array_equal(qc_gpu(ctx, x, ones, n), x[:n] ^ tot)
 
 
 # 4096 and 2^16 take the word-aligned path (n a multiple of 32, >= 64 words)

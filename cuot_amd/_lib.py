@@ -62,7 +62,8 @@ class ServerResult(C.Structure):
 
 
 class KnobValues(C.Structure):
-    _fields_ = [("qc_radix", I32), ("qc_cols", I32), ("ggm_dfs", I32), ("lpn_wgs", I32), ("qc_mfma_max", I32)]
+    _fields_ = [("qc_radix", I32), ("qc_cols", I32), ("ggm_dfs", I32), ("lpn_wgs", I32), ("qc_mfma_max", I32),
+                ("aes_impl", I32)]
 
 
 class LaunchTime(C.Structure):

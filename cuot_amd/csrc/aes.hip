@@ -132,6 +132,7 @@ static unsigned stream_grid(int64_t n) {
 
 hipError_t cuot_launch_aes_ecb(const uint32_t rk[44], const blk* in, blk* out, int64_t n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
+  if (cuot_knobs().aes_impl == 2) return cuot_launch_aes_ecb_bs(rk, in, out, n, st);
   EcbArgs a;
   a.T0 = cuot_device_T0();
   memcpy(a.rk, rk, sizeof(a.rk));
@@ -139,13 +140,16 @@ hipError_t cuot_launch_aes_ecb(const uint32_t rk[44], const blk* in, blk* out, i
   a.out = out;
   a.n = n;
   a.stream = a.start = 0;
+  cuot_rec_begin(st, "aes_stream", 32.0 * (double)n);
   hipLaunchKernelGGL(aes_stream_kernel<false>, dim3(stream_grid(n)), dim3(256), 0, st, a);
+  cuot_rec_end(st);
   return hipGetLastError();
 }
 
 hipError_t cuot_launch_prg(const uint32_t rk[44], uint64_t stream, uint64_t start, int64_t n, blk* out,
                            hipStream_t st) {
   if (n <= 0) return hipSuccess;
+  if (cuot_knobs().aes_impl == 2) return cuot_launch_prg_bs(rk, stream, start, n, out, st);
   EcbArgs a;
   a.T0 = cuot_device_T0();
   memcpy(a.rk, rk, sizeof(a.rk));
@@ -154,7 +158,9 @@ hipError_t cuot_launch_prg(const uint32_t rk[44], uint64_t stream, uint64_t star
   a.n = n;
   a.stream = stream;
   a.start = start;
+  cuot_rec_begin(st, "aes_stream", 16.0 * (double)n);
   hipLaunchKernelGGL(aes_stream_kernel<true>, dim3(stream_grid(n)), dim3(256), 0, st, a);
+  cuot_rec_end(st);
   return hipGetLastError();
 }
 

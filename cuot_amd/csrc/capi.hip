@@ -263,7 +263,7 @@ extern "C" int cuot_bit_untranspose(cuot_ctx* ctx, const uint8_t* rows_d, int64_
 
 // ---- execution knobs ---------------------------------------------------------
 namespace {
-CuotKnobs g_knobs = {0, 0, 0, 0, 0};
+CuotKnobs g_knobs = {0, 0, 0, 0, 0, 0};
 }
 
 const CuotKnobs& cuot_knobs() { return g_knobs; }
@@ -276,11 +276,13 @@ extern "C" int cuot_set_knobs(const cuot_knob_values* k) {
   CUOT_CHECK_ARG(k->ggm_dfs >= 0 && k->ggm_dfs <= 6, "ggm_dfs must be in [0, 6]");
   CUOT_CHECK_ARG(k->lpn_wgs >= 0, "lpn_wgs must be >= 0");
   CUOT_CHECK_ARG(k->qc_mfma_max >= -1, "qc_mfma_max must be >= -1");
+  CUOT_CHECK_ARG(k->aes_impl >= 0 && k->aes_impl <= 2, "aes_impl must be 0, 1 or 2");
   g_knobs.qc_radix = k->qc_radix;
   g_knobs.qc_cols = k->qc_cols;
   g_knobs.ggm_dfs = k->ggm_dfs;
   g_knobs.lpn_wgs = k->lpn_wgs;
   g_knobs.qc_mfma_max = k->qc_mfma_max;
+  g_knobs.aes_impl = k->aes_impl;
   return CUOT_OK;
 }
 
@@ -291,6 +293,7 @@ extern "C" int cuot_get_knobs(cuot_knob_values* k) {
   k->ggm_dfs = g_knobs.ggm_dfs;
   k->lpn_wgs = g_knobs.lpn_wgs;
   k->qc_mfma_max = g_knobs.qc_mfma_max;
+  k->aes_impl = g_knobs.aes_impl;
   return CUOT_OK;
 }
 

@@ -34,6 +34,10 @@ hipError_t cuot_launch_aes_ecb(const uint32_t rk[44], const blk* in, blk* out, i
 // out[i] = AES_key({lo = start + i, hi = stream})
 hipError_t cuot_launch_prg(const uint32_t rk[44], uint64_t stream, uint64_t start, int64_t n, blk* out,
                            hipStream_t st);
+// the same two streams with the bitsliced cipher (aes_bs.hip)
+hipError_t cuot_launch_aes_ecb_bs(const uint32_t rk[44], const blk* in, blk* out, int64_t n, hipStream_t st);
+hipError_t cuot_launch_prg_bs(const uint32_t rk[44], uint64_t stream, uint64_t start, int64_t n, blk* out,
+                              hipStream_t st);
 
 // Host helpers (aes_host.cpp).
 void cuot_aes_expand_host(const uint8_t key[16], uint32_t rk[44]);
@@ -132,6 +136,7 @@ struct CuotKnobs {
   int ggm_dfs;   // per-lane depth-first levels L (h - 10 <= L <= min(h - 8, 6))
   int lpn_wgs;   // LPN grid cap
   int qc_mfma_max;  // matrix-core QC encoder for n at or below (-1: never)
+  int aes_impl;     // AES streams: 0 rule / 1 LDS T-table / 2 bitsliced
 };
 const CuotKnobs& cuot_knobs();
 

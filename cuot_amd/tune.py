@@ -131,8 +131,8 @@ class CostModel:
 
 
 # ---- knobs -----------------------------------------------------------------------
-def set_knobs(qc_radix=0, qc_cols=0, ggm_dfs=0, lpn_wgs=0, qc_mfma_max=0):
-    k = KnobValues(qc_radix, qc_cols, ggm_dfs, lpn_wgs, qc_mfma_max)
+def set_knobs(qc_radix=0, qc_cols=0, ggm_dfs=0, lpn_wgs=0, qc_mfma_max=0, aes_impl=0):
+    k = KnobValues(qc_radix, qc_cols, ggm_dfs, lpn_wgs, qc_mfma_max, aes_impl)
     check(_lib.load().cuot_set_knobs(C.byref(k)), "cuot_set_knobs")
 
 
@@ -140,7 +140,7 @@ def get_knobs() -> dict:
     k = KnobValues()
     check(_lib.load().cuot_get_knobs(C.byref(k)), "cuot_get_knobs")
     return {"qc_radix": k.qc_radix, "qc_cols": k.qc_cols, "ggm_dfs": k.ggm_dfs, "lpn_wgs": k.lpn_wgs,
-            "qc_mfma_max": k.qc_mfma_max}
+            "qc_mfma_max": k.qc_mfma_max, "aes_impl": k.aes_impl}
 
 
 QC_SPACE = [dict(qc_radix=r, qc_cols=c) for r, c in itertools.product((4, 5, 6, 7), (1, 2, 4, 8))]

@@ -426,6 +426,7 @@ typedef struct {
   int32_t lpn_wgs;  /* LPN grid cap */
   int32_t qc_mfma_max; /* QC encodes with 17 <= n <= this run as a 0/1 int8 matrix-core product
                           (at most 2^16; -1: never, 0: the built-in crossover) */
+  int32_t aes_impl;    /* cuot_aes_ecb / cuot_prg cipher: 0 rule (T-table), 1 LDS T-table, 2 bitsliced */
 } cuot_knob_values;
 int cuot_set_knobs(const cuot_knob_values *k);
 int cuot_get_knobs(cuot_knob_values *k);

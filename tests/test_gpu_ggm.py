@@ -30,6 +30,46 @@ def test_aes_fips197_vector(ctx):
     assert po.block_to_bytes(to_np(yd)).hex() == "69c4e0d86a7b0430d8cdb78070b4c55a"
 
 
+@pytest.fixture
+def aes_impl():
+    """Selects the AES stream cipher variant for one test, restoring the rule after."""
+    from cuot_amd import tune
+
+    def set_(v):
+        tune.set_knobs(aes_impl=v)
+
+    yield set_
+    tune.set_knobs()
+
+
+# the bitsliced cipher (32 blocks per lane, 2048 per wave): ragged waves and
+# workgroups, the FIPS-197 answer, and counter mode
+@pytest.mark.parametrize("n", [1, 31, 2047, 2048, 2049, 8193, 70001])
+def test_aes_bitsliced_matches_oracle(ctx, aes_impl, n):
+    print(f"aes bitsliced n={n}", flush=True)
+    aes_impl(2)
+    r = rng(50 + n)
+    key = rand_blocks(r, 1)
+    x = rand_blocks(r, n)
+    xd, yd = to_dev(x), dev_blocks(n)
+    ctx.aes_ecb(key[0], xd, yd)
+    assert np.array_equal(to_np(yd)[:n], po.aes_ecb(key, x))
+
+
+def test_aes_bitsliced_fips197_and_prg(ctx, aes_impl):
+    aes_impl(2)
+    key = np.frombuffer(bytes(range(16)), dtype=np.uint64)
+    x = po.block_from_bytes(bytes.fromhex("00112233445566778899aabbccddeeff"))
+    yd = dev_blocks(1)
+    ctx.aes_ecb(key, to_dev(x), yd)
+    assert po.block_to_bytes(to_np(yd)).hex() == "69c4e0d86a7b0430d8cdb78070b4c55a"
+    r = rng(3)
+    seed = rand_blocks(r, 1)
+    yd = dev_blocks(5000)
+    ctx.prg(seed[0], 3, 77, yd)
+    assert np.array_equal(to_np(yd), po.prg(seed, 3, 77, 5000))
+
+
 def test_prg_matches_oracle(ctx):
     r = rng(2)
     seed = rand_blocks(r, 1)

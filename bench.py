@@ -53,7 +53,197 @@ def parse():
     ap.add_argument("--qc-only", action="store_true",
                     help="time only the quasi-cyclic encode of one party at n = 2^sot-n-log (after a 256 MiB "
                          "device copy that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE) and print one JSON line")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the two rocprofv3 --pmc passes that measure HBM traffic and VALU work per kernel")
+    ap.add_argument("--counter-run", action="store_true",
+                    help="(used under rocprofv3 by the counter passes) one QC encode at 2^24, one Ferret run, "
+                         "AES-128 ECB of 2^24 blocks with both ciphers and one matrix-core QC encode; prints the "
+                         "kernels' algorithmic bytes as one JSON line")
     return ap.parse_args()
+
+
+# ---- hardware counters -------------------------------------------------------------
+VALU_LANE_OPS_PER_S = 256 * 64 * 2.4e9  # one wave64 VALU instruction per CU per cycle (issue model)
+COUNTER_PASSES = (
+    ["FETCH_SIZE", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES",
+     "SQ_WAVE_CYCLES"],
+    ["WRITE_SIZE"],
+)
+CALIB_BYTES = 1 << 28  # the device copy that calibrates FETCH_SIZE / WRITE_SIZE
+
+
+def progress(msg: str):
+    """One line on stderr as each leg finishes (the JSON line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_base(name: str) -> tuple[str, str]:
+    """(base name, name with template arguments) of a rocprofv3 kernel name."""
+    s = name.replace("(anonymous namespace)::", "").strip()
+    if s.startswith("void "):
+        s = s[5:]
+    full = s.split("(")[0].strip()
+    return full.split("<")[0], full
+
+
+def counter_run(args):
+    """The counter passes' workload (runs under rocprofv3): a 256 MiB device
+    copy first (a kernel of known traffic), then the kernels the bench line
+    reports counters for.  Prints {kernel: {launches, bytes}} from the
+    library's launch recorder (algorithmic bytes)."""
+    import ctypes as C
+
+    import torch
+
+    import cuot_amd
+    from cuot_amd import tune
+
+    ctx = cuot_amd.Context(0)
+    lib = ctx.lib
+    dev = "cuda"
+    src = torch.empty(CALIB_BYTES, dtype=torch.uint8, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    acc = {}
+
+    def timed(fn):
+        cuot_amd.check(lib.cuot_timing_begin(ctx.handle), "cuot_timing_begin")
+        fn()
+        recs = (cuot_amd._lib.LaunchTime * 4096)()
+        nrec = C.c_int()
+        cuot_amd.check(lib.cuot_timing_end(ctx.handle, recs, 4096, C.byref(nrec)), "cuot_timing_end")
+        for r in recs[: nrec.value]:
+            k = acc.setdefault(r.kernel.decode().split("<")[0], {"launches": 0, "bytes": 0.0})
+            k["launches"] += 1
+            k["bytes"] += r.bytes
+
+    g = torch.Generator().manual_seed(1)
+    n = 1 << 24
+    a_bits = torch.randint(0, 256, ((n + 7) // 8 + 16,), dtype=torch.uint8, generator=g).to(dev)
+    ahat = torch.empty(ctx.qc_ahat_len(n), dtype=torch.int32, device=dev)
+    scratch = torch.empty(ctx.qc_scratch_bytes(n), dtype=torch.uint8, device=dev)
+    x = torch.randint(-2**62, 2**62, (2 * n, 2), dtype=torch.int64, generator=g).to(dev)
+    y = torch.empty((n, 2), dtype=torch.int64, device=dev)
+    ctx.qc_prepare(n, a_bits, ahat, scratch)
+    timed(lambda: ctx.qc_encode(n, ahat, x, y, scratch))
+    del a_bits, ahat, scratch, x
+    B = cuot_amd.to_block
+    fn, fk, ft, fh = 10168320, 452160, 1270, 13
+    fcfg = cuot_amd.FerretConfig(n=fn, k=fk, t=ft, h=fh, ntasks=1, seed=B(SEED))
+    fws = torch.empty(ctx.ferret_workspace_bytes(fcfg), dtype=torch.uint8, device=dev)
+    fs = torch.empty((fn, 2), dtype=torch.int64, device=dev)
+    fr = torch.empty((fn, 2), dtype=torch.int64, device=dev)
+    fd = torch.empty((1, 2), dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+    timed(lambda: ctx.ferret_run(fcfg, fws, fs, fr, fd, stream))
+    del fws, fs, fr
+    key = (0x0123456789ABCDEF, 0x0FEDCBA987654321)
+    for impl in (1, 2):
+        tune.set_knobs(aes_impl=impl)
+        timed(lambda: ctx.aes_ecb(key, y, y))
+    tune.set_knobs()
+    del y
+    m = 32768
+    a_bits = torch.randint(0, 256, ((m + 31) // 32 * 4,), dtype=torch.uint8, generator=g).to(dev)
+    ahat = torch.empty(ctx.qc_ahat_len(m), dtype=torch.int32, device=dev)
+    scratch = torch.empty(ctx.qc_scratch_bytes(m), dtype=torch.uint8, device=dev)
+    x = torch.randint(-2**62, 2**62, (2 * m, 2), dtype=torch.int64, generator=g).to(dev)
+    y = torch.empty((m, 2), dtype=torch.int64, device=dev)
+    ctx.qc_prepare(m, a_bits, ahat, scratch)
+    tune.set_knobs(qc_mfma_max=1 << 16)
+    timed(lambda: ctx.qc_encode(m, ahat, x, y, scratch))
+    tune.set_knobs()
+    torch.cuda.synchronize()
+    print(json.dumps({"counter_run": acc}), flush=True)
+    ctx.close()
+    return 0
+
+
+def collect_counters():
+    """Two rocprofv3 --pmc passes over the counter-run workload, started
+    before this process touches the GPU.  Per kernel (averaged over its
+    dispatches): HBM bytes (FETCH_SIZE, WRITE_SIZE in KiB, scaled by what the
+    counters read for the 256 MiB copy), VALU and LDS instructions, LDS bank
+    conflicts, the profiled duration; plus the algorithmic bytes per launch
+    the library's recorder notes.  Returns (table, note)."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, "rocprofv3 not on PATH"
+    disp = {}  # dispatch id -> {"name", "ns", counters}
+    algo = None
+    for pi, ctrs in enumerate(COUNTER_PASSES):
+        d = tempfile.mkdtemp(prefix="cuot_pmc_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", "180", exe, "--pmc", *ctrs, "--output-format", "csv", "-d", d, "-o", "run",
+               "--", sys.executable, os.path.join(ROOT, "bench.py"), "--counter-run"]
+        env = dict(os.environ, TMPDIR="/tmp", PYTHONPATH=ROOT)
+        progress(f"counter pass {pi + 1}/{len(COUNTER_PASSES)}: {' '.join(ctrs)}")
+        r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True)
+        for line in r.stdout.splitlines():
+            if line.startswith('{"counter_run"'):
+                algo = json.loads(line)["counter_run"]
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            return None, f"counter pass {pi + 1} wrote no counter file (exit {r.returncode})"
+        for row in csv.DictReader(open(files[0])):
+            key = (pi, row["Dispatch_Id"])
+            e = disp.setdefault(key, {"name": row["Kernel_Name"],
+                                      "ns": float(row["End_Timestamp"]) - float(row["Start_Timestamp"])})
+            e[row["Counter_Name"]] = e.get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+        shutil.rmtree(d, ignore_errors=True)
+
+    def calib(counter, pi):
+        vals = [e[counter] for (p, _), e in disp.items() if p == pi and "copyBuffer" in e["name"] and counter in e]
+        return CALIB_BYTES / (max(vals) * 1024.0) if vals and max(vals) > 0 else None
+
+    fscale, wscale = calib("FETCH_SIZE", 0), calib("WRITE_SIZE", 1)
+    table = {}
+    for (pi, _), e in disp.items():
+        if "copyBuffer" in e["name"] or "rocclr" in e["name"]:
+            continue
+        base, full = kernel_base(e["name"])
+        k = table.setdefault(full, {"base": base, "n0": 0, "n1": 0, "ns": 0.0, "fetch": 0.0, "write": 0.0,
+                                    "valu": 0.0, "lds": 0.0, "conflict": 0.0})
+        if pi == 0:
+            k["n0"] += 1
+            k["ns"] += e["ns"]
+            k["fetch"] += e.get("FETCH_SIZE", 0.0)
+            k["valu"] += e.get("SQ_INSTS_VALU", 0.0)
+            k["lds"] += e.get("SQ_INSTS_LDS", 0.0)
+            k["conflict"] += e.get("SQ_LDS_BANK_CONFLICT", 0.0)
+        else:
+            k["n1"] += 1
+            k["write"] += e.get("WRITE_SIZE", 0.0)
+    out = {}
+    for full, k in table.items():
+        if not k["n0"]:
+            continue
+        us = k["ns"] / k["n0"] / 1e3
+        row = {"launches": k["n0"], "us_per_launch": round(us, 1),
+               "valu_instr_per_launch": round(k["valu"] / k["n0"]),
+               "valu_util": round(k["valu"] / k["n0"] * 64 / (us * 1e-6 * VALU_LANE_OPS_PER_S), 3) if us > 0 else None,
+               "lds_instr_per_launch": round(k["lds"] / k["n0"]),
+               "lds_bank_conflict_cycles_per_launch": round(k["conflict"] / k["n0"])}
+        if fscale and wscale and k["n1"]:
+            tb = k["fetch"] / k["n0"] * 1024 * fscale + k["write"] / k["n1"] * 1024 * wscale
+            row["hbm_bytes_per_launch"] = round(tb)
+            a = (algo or {}).get(k["base"])
+            if a and a["launches"] and a["bytes"] > 0:
+                row["algorithmic_bytes_per_launch"] = round(a["bytes"] / a["launches"])
+                row["traffic_over_algorithmic"] = round(tb / (a["bytes"] / a["launches"]), 3)
+        out[full] = row
+    note = (f"rocprofv3 --pmc, two passes over bench.py --counter-run; FETCH_SIZE x {fscale:.3f}, "
+            f"WRITE_SIZE x {wscale:.3f} (KiB -> bytes, from the 256 MiB copy); VALU util = VALU instructions x 64 "
+            f"/ (profiled duration x {VALU_LANE_OPS_PER_S:.3g} lane-ops/s)") if fscale and wscale else \
+        "copy kernel not found: no byte calibration"
+    return out, note
 
 
 def qc_only(args):
@@ -154,20 +344,104 @@ def cpu_baseline(threads: int):
     }
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic.md")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        parts = [p.strip() for p in line.strip().strip("|").split("|")]
-        if len(parts) >= 4 and parts[0] == kernel:
-            try:
-                return float(parts[3])
-            except ValueError:
-                return None
-    return None
+def timed_kernels(cuot_amd, ctx, fn, reps=3):
+    """Minimum over `reps` calls of fn of the summed per-launch times (HIP
+    events around every launch), with the per-kernel split of that call."""
+    import ctypes as C
+
+    lib = ctx.lib
+    best = None
+    for _ in range(reps):
+        cuot_amd.check(lib.cuot_timing_begin(ctx.handle), "cuot_timing_begin")
+        fn()
+        recs = (cuot_amd._lib.LaunchTime * 1024)()
+        nrec = C.c_int()
+        cuot_amd.check(lib.cuot_timing_end(ctx.handle, recs, 1024, C.byref(nrec)), "cuot_timing_end")
+        rows = [(recs[i].kernel.decode(), recs[i].ms) for i in range(nrec.value)]
+        tot = sum(ms for _, ms in rows)
+        if best is None or tot < best[0]:
+            best = (tot, rows)
+    split = {}
+    for k, ms in best[1]:
+        split[k] = round(split.get(k, 0.0) + ms, 4)
+    return best[0], split
+
+
+MFMA_I8_PEAK_TOPS = 5000.0  # MI355X dense int8
+
+
+def small_code_leg(torch, cuot_amd, ctx, counters):
+    """QC encode of small codes: the additive FFT against the int8 matrix-core
+    product on one prepared code per size (both exact; outputs compared), the
+    matrix cores' int8 rate against the dense peak, and the crossover the
+    library uses."""
+    from cuot_amd import tune
+
+    g = torch.Generator().manual_seed(5)
+    rows = []
+    for n in (4096, 16384, 32768, 49152, 65536):
+        a_bits = torch.randint(0, 256, ((n + 31) // 32 * 4,), dtype=torch.uint8, generator=g).cuda()
+        ahat = torch.empty(ctx.qc_ahat_len(n), dtype=torch.int32, device="cuda")
+        scratch = torch.empty(ctx.qc_scratch_bytes(n), dtype=torch.uint8, device="cuda")
+        x = torch.randint(-2**62, 2**62, (2 * n, 2), dtype=torch.int64, generator=g).cuda()
+        y = torch.empty((n, 2), dtype=torch.int64, device="cuda")
+        ctx.qc_prepare(n, a_bits, ahat, scratch)
+        res, outs = {}, {}
+        for name, lim in (("fft", -1), ("mfma", 1 << 16)):
+            tune.set_knobs(qc_mfma_max=lim)
+            ctx.qc_encode(n, ahat, x, y, scratch)
+            torch.cuda.synchronize()
+            outs[name] = y.clone()
+            res[name] = timed_kernels(cuot_amd, ctx, lambda: ctx.qc_encode(n, ahat, x, y, scratch))
+        tune.set_knobs()
+        ops = 2.0 * 128 * n * n
+        enc = res["mfma"][1].get("qc_mf_encode", res["mfma"][0])
+        rows.append({"n": n, "fft_us": round(res["fft"][0] * 1e3, 1), "mfma_us": round(res["mfma"][0] * 1e3, 1),
+                     "mfma_split_us": {k: round(v * 1e3, 1) for k, v in res["mfma"][1].items()},
+                     "mfma_int8_TOPs": round(ops / (enc * 1e-3) / 1e12, 1),
+                     "mfma_frac_of_dense_peak": round(ops / (enc * 1e-3) / 1e12 / MFMA_I8_PEAK_TOPS, 4),
+                     "faster": "mfma" if res["mfma"][0] < res["fft"][0] else "fft",
+                     "same_output": bool(torch.equal(outs["fft"], outs["mfma"]))})
+        del a_bits, ahat, scratch, x, y
+    mf = next((v for k, v in (counters or {}).items() if k.startswith("qc_mf_encode")), None)
+    return {"workload": "QC encode of one party, small n, both encoders on one prepared code",
+            "sizes": rows, "builtin_crossover_n": 43000,
+            "crossover_rule": "matrix cores for 17 <= n <= crossover (knob qc_mfma_max), additive FFT above",
+            "roofline_mfma": {"bound": "mfma", "kernel": "qc_mf_encode", "unit": "TOP/s", "peak": MFMA_I8_PEAK_TOPS,
+                              "achieved": rows[-1]["mfma_int8_TOPs"], "frac": rows[-1]["mfma_frac_of_dense_peak"],
+                              "n": rows[-1]["n"]},
+            "counters_n32768": mf}
+
+
+def aes_leg(torch, cuot_amd, ctx, counters):
+    """AES-128 ECB of 2^24 blocks with the LDS T-table and the bitsliced
+    cipher (same output): AES blocks/s and VALU lane-ops per block (counter
+    pass), and which one the library uses."""
+    from cuot_amd import tune
+
+    n = 1 << 24
+    g = torch.Generator().manual_seed(6)
+    x = torch.randint(-2**62, 2**62, (n, 2), dtype=torch.int64, generator=g).cuda()
+    y = torch.empty_like(x)
+    key = (0x0123456789ABCDEF, 0x0FEDCBA987654321)
+    out, outs = {}, {}
+    for name, impl, kname in (("ttable", 1, "aes_stream_kernel<false>"), ("bitsliced", 2, "aes_bs_stream_kernel<false>")):
+        tune.set_knobs(aes_impl=impl)
+        ctx.aes_ecb(key, x, y)
+        torch.cuda.synchronize()
+        outs[name] = y.clone()
+        ms, _ = timed_kernels(cuot_amd, ctx, lambda: ctx.aes_ecb(key, x, y))
+        row = (counters or {}).get(kname)
+        out[name] = {"ms": round(ms, 4), "aes_per_s": round(n / (ms * 1e-3), 1),
+                     "valu_lane_ops_per_aes": round(row["valu_instr_per_launch"] * 64 / n, 1) if row else None,
+                     "lds_lane_ops_per_aes": round(row["lds_instr_per_launch"] * 64 / n, 1) if row else None,
+                     "valu_util": row.get("valu_util") if row else None}
+    tune.set_knobs()
+    out["same_output"] = bool(torch.equal(outs["ttable"], outs["bitsliced"]))
+    out["faster"] = max(("ttable", "bitsliced"), key=lambda k: out[k]["aes_per_s"])
+    out["library_rule"] = "ttable"
+    out["blocks"] = n
+    return out
 
 
 def tune_for_mix(ctx, mix):
@@ -414,6 +688,14 @@ def main():
         return 2
     if args.qc_only:
         return qc_only(args)
+    if args.counter_run:
+        return counter_run(args)
+    counters, counters_note = (None, "skipped (--no-counters)")
+    if not args.no_counters:
+        try:
+            counters, counters_note = collect_counters()
+        except Exception as e:  # the counters are a report, never a reason to lose the bench line
+            counters, counters_note = None, f"counter passes failed: {type(e).__name__}: {e}"
     import torch
 
     import cuot_amd
@@ -490,11 +772,24 @@ def main():
     sot_whole()
     torch.cuda.synchronize()
     ok = relation_ok(torch, v, w, b, d, n)
+    progress(f"SOT leg done: {ms_per_step:.3f} ms per instance")
 
     qc_ms = (stage_ms["qc_sender"] + stage_ms["qc_recver"]) / 2
     qc_bytes = 48 * n  # compulsory per encode: read 2n blocks, write n blocks
     dom_achieved = dom["bytes"] / (dom["ms"] / 1e3) / 1e9
-    traffic = measured_traffic(dom_name)
+    # the dominant kernel's counters, averaged over its template variants by launches
+    variants = [v for k, v in (counters or {}).items() if kernel_base(k)[0] == dom_name]
+    nl = sum(v["launches"] for v in variants)
+    dom_ctr = None
+    if nl:
+        dom_ctr = {"valu_instr_per_launch": round(sum(v["valu_instr_per_launch"] * v["launches"] for v in variants) / nl)}
+        if all("hbm_bytes_per_launch" in v for v in variants):
+            dom_ctr["hbm_bytes_per_launch"] = round(sum(v["hbm_bytes_per_launch"] * v["launches"] for v in variants)
+                                                    / nl)
+    traffic = dom_ctr.get("hbm_bytes_per_launch") if dom_ctr else None
+    dom_launch_ms = dom["ms"] / dom["launches"]
+    dom_valu = (dom_ctr["valu_instr_per_launch"] * 64 / (dom_launch_ms * 1e-3 * VALU_LANE_OPS_PER_S)
+                if dom_ctr and dom_ctr.get("valu_instr_per_launch") else None)
     ggm_ms = (stage_ms["ggm_sender"] + stage_ms["ggm_recver"]) / 2
     aes_per_party = 2 * t * ((1 << h) - 1)
 
@@ -521,6 +816,23 @@ def main():
         torch.cuda.synchronize()
         f1 = time.perf_counter()
         fms = (f1 - f0) * 1e3 / args.steps
+        # per-kernel launch times of one more staged run (events around each launch)
+        cuot_amd.check(lib.cuot_timing_begin(ctx.handle), "cuot_timing_begin")
+        for i, (_, bit) in enumerate(fst):
+            ctx.ferret_run_stages(fcfg, fws, fs, fr, fd, bit, stream)
+        frecs = (cuot_amd._lib.LaunchTime * 4096)()
+        fnrec = C.c_int()
+        cuot_amd.check(lib.cuot_timing_end(ctx.handle, frecs, 4096, C.byref(fnrec)), "cuot_timing_end")
+        fk_ = {}
+        for r in frecs[: fnrec.value]:
+            k = fk_.setdefault(r.kernel.decode(), {"launches": 0, "ms": 0.0, "bytes": 0.0})
+            k["launches"] += 1
+            k["ms"] += r.ms
+            k["bytes"] += r.bytes
+        ferret_kernels = {name: {"launches": k["launches"], "ms": round(k["ms"], 4),
+                                 "algorithmic_GBps": round(k["bytes"] / (k["ms"] / 1e3) / 1e9, 1) if k["ms"] else None,
+                                 "algorithmic_bytes_per_launch": round(k["bytes"] / k["launches"])}
+                          for name, k in fk_.items()}
         lsb = (fr[:, 0] & 1).to(torch.bool)
         fok = bool(torch.equal(torch.where(lsb.unsqueeze(1), fs ^ fd, fs), fr))
         ferret = {
@@ -531,7 +843,21 @@ def main():
             "stage_ms": {name: round(sum(fev[s][i].elapsed_time(fev[s][i + 1]) for s in range(args.steps))
                                      / args.steps, 4) for i, (name, _) in enumerate(fst)},
             "relation_ok": fok,
+            "kernels": ferret_kernels,
         }
+        lk = ferret_kernels.get("lpn_kernel")
+        if lk:
+            lpn_row = next((v for k, v in (counters or {}).items() if k.startswith("lpn_kernel")), None)
+            ferret["lpn"] = {
+                "ms_per_launch": round(lk["ms"] / lk["launches"], 4),
+                "algorithmic_GBps": lk["algorithmic_GBps"],
+                "hbm_frac_algorithmic": round(lk["algorithmic_GBps"] / HBM_PEAK_GBS, 4),
+                "algorithmic_bytes_per_launch": lk["algorithmic_bytes_per_launch"],
+                "bytes_per_row_per_role": 16 * 12,
+                "hbm_bytes_per_launch": lpn_row.get("hbm_bytes_per_launch") if lpn_row else None,
+                "traffic_over_algorithmic": lpn_row.get("traffic_over_algorithmic") if lpn_row else None,
+                "valu_util": lpn_row.get("valu_util") if lpn_row else None,
+            }
 
     # iterating Ferret session (FerretCOT extend loop) at the C2 size
     session = None
@@ -613,6 +939,11 @@ def main():
     # release the headline buffers before the other legs
     del ws, v, w, b
     torch.cuda.empty_cache()
+    small = small_code_leg(torch, cuot_amd, ctx, counters)
+    progress("small-code QC leg done")
+    aes = aes_leg(torch, cuot_amd, ctx, counters)
+    progress("AES leg done")
+    torch.cuda.empty_cache()
     serving = None
     batched = None
     rates = [float(r) for r in args.serve_sweep.split(",") if r.strip()]
@@ -623,9 +954,11 @@ def main():
         serving = serve_leg(torch, cuot_amd, ctx, args.serve_requests, args.serve_rate, args.serve_budget_gb, rates,
                             args.sweep_requests)
         serving["leg_order"] = "batched leg first" if (args.batch_first and not args.no_batch) else "serving first"
+        progress("serving leg done")
     if not args.batch_first and not args.no_batch:
         batched = batch_leg(torch, cuot_amd, ctx, max(1, args.steps))
         torch.cuda.empty_cache()
+        progress("batched leg done")
 
     cpu = None
     if not args.no_cpu:
@@ -635,6 +968,7 @@ def main():
         cores = len(os.sched_getaffinity(0))
         threads = args.cpu_threads or (min(cores, share) if share > 0 else cores)
         cpu = cpu_baseline(threads)
+        progress("CPU baseline done")
 
     value = n / (ms_per_step / 1e3)
     out = {
@@ -656,21 +990,27 @@ def main():
         "relation_ok": ok,
         "stage_ms": stage_ms,
         "roofline": {
-            "bound": "hbm",
+            "bound": "valu" if dom_valu is not None and dom_valu > dom_achieved / HBM_PEAK_GBS else "hbm",
             "kernel": dom_name,
             "achieved": round(dom_achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(dom_achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "avg_launch_ms": round(dom["ms"] / dom["launches"], 4),
+            "avg_launch_ms": round(dom_launch_ms, 4),
             "algorithmic_bytes_per_launch": round(dom["bytes"] / dom["launches"]),
+            "valu_frac": round(dom_valu, 4) if dom_valu is not None else None,
+            "valu_instr_per_launch": dom_ctr.get("valu_instr_per_launch") if dom_ctr else None,
+            "valu_peak_lane_ops_per_s": VALU_LANE_OPS_PER_S,
         },
+        "counters": {"note": counters_note, "kernels": counters},
         "kernels": per_kernel,
         "qc_encode": {"ms_per_party": round(qc_ms, 4), "compulsory_bytes": qc_bytes,
                       "compulsory_GBps": round(qc_bytes / (qc_ms / 1e3) / 1e9, 1)},
         "ggm": {"aes_per_launch": aes_per_party, "ms_per_launch": round(ggm_ms, 4),
                 "aes_per_s": round(aes_per_party / (ggm_ms / 1e3), 1)},
+        "qc_small_codes": small,
+        "aes": aes,
         "ferret": ferret,
         "ferret_session": session,
         "batched": batched,

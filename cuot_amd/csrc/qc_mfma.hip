@@ -282,7 +282,8 @@ hipError_t cuot_launch_qc_mfma(const QcIo* io, QcIo io1, const uint32_t* bits1, 
   hipLaunchKernelGGL(qc_mf_unpack, dim3((unsigned)((uthreads + 255) / 256), (unsigned)B), dim3(256), 0, st, io, io1,
                      (uint4*)scratch, e8_stride);
   cuot_rec_end(st);
-  cuot_rec_begin(st, "qc_mf_encode", 2.0 * 128.0 * (double)nmax * (double)pl.k32 * B);
+  // algorithmic bytes: E8 read once, y read and written
+  cuot_rec_begin(st, "qc_mf_encode", (128.0 * (double)pl.k32 + 32.0 * (double)nmax) * B);
   hipLaunchKernelGGL(qc_mf_encode, dim3((unsigned)pl.gx, (unsigned)S, (unsigned)B), dim3(256), 0, st, io, io1, bits1,
                      ahats, (int64_t)1 << m, (const uint4*)scratch, e8_stride);
   cuot_rec_end(st);

@@ -2,8 +2,9 @@
 are refused with CUOT_ERR_ARG before any launch, the serving loop rejects a
 request larger than its whole HBM budget (CUOT_ERR_NOMEM on that request
 only), defers a request while the arena is full and admits it once memory is
-released, and refuses to release a request that is still queued
-(CUOT_ERR_BUSY).  Every kernel that runs here exits normally."""
+released, refuses to release a request that is still queued
+(CUOT_ERR_BUSY), and refuses requests for resident clients it does not have.
+Every kernel that runs here exits normally."""
 import ctypes as C
 import time
 
@@ -97,3 +98,21 @@ def test_server_budget_refusal_deferral_busy(ctx):
     s, rr, d = run_ferret_gpu(ctx, small_wi.n, small_wi.k, small_wi.t, small_wi.h,
                               np.array(small_wi.seed, dtype=np.uint64))
     assert np.array_equal(a[0], to_np(s)) and np.array_equal(a[1], to_np(rr)) and np.array_equal(a[3], to_np(d)[0])
+
+
+def test_server_refuses_unknown_client(ctx):
+    import cuot_amd
+    from cuot_amd.workloads import FERRET, SOT, make_instance
+
+    wi = make_instance(FERRET, 50000, (9, 10))
+    srv = cuot_amd.Server(ctx, hbm_budget=1 << 26, max_batch=1, streams=1)
+    try:
+        for kind, client in ((FERRET, 1), (SOT, 1), (FERRET, -1)):
+            inst = cuot_amd.make_instance_struct(kind, wi.n, wi.t, wi.h, wi.k, (9, 10), k0=SOT_K0, k1=SOT_K1,
+                                                 client=client)
+            t = C.c_int64()
+            assert ctx.lib.cuot_server_submit(srv.handle, C.byref(inst), C.byref(t)) == ERR_ARG
+        nb, used = srv.stats()
+        assert used == 0
+    finally:
+        srv.close()

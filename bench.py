@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--qc-only", action="store_true",
                     help="time only the quasi-cyclic encode of one party at n = 2^sot-n-log (after a 256 MiB "
                          "device copy that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE) and print one JSON line")
+    ap.add_argument("--no-small-codes", action="store_true", help="skip the small-code QC comparison leg")
+    ap.add_argument("--no-aes", action="store_true", help="skip the AES cipher comparison leg")
     ap.add_argument("--no-counters", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure HBM traffic and VALU work per kernel")
     ap.add_argument("--counter-run", action="store_true",
@@ -939,10 +941,13 @@ def main():
     # release the headline buffers before the other legs
     del ws, v, w, b
     torch.cuda.empty_cache()
-    small = small_code_leg(torch, cuot_amd, ctx, counters)
-    progress("small-code QC leg done")
-    aes = aes_leg(torch, cuot_amd, ctx, counters)
-    progress("AES leg done")
+    small = aes = None
+    if not args.no_small_codes:
+        small = small_code_leg(torch, cuot_amd, ctx, counters)
+        progress("small-code QC leg done")
+    if not args.no_aes:
+        aes = aes_leg(torch, cuot_amd, ctx, counters)
+        progress("AES leg done")
     torch.cuda.empty_cache()
     serving = None
     batched = None

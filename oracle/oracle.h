@@ -28,7 +28,7 @@
  *   or_bit_transpose involution and per-bit brute force.
  *   emp-tool internals (makeBlock order, ParaEnc key assignment) are read
  *   from call sites only: parity unpinned against emp-tool itself, which is
- *   not present.
+ *   not present in the reference tree (emp-tool @44b1dde, not vendored).
  */
 #ifndef CUOT_ORACLE_H
 #define CUOT_ORACLE_H

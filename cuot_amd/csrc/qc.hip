@@ -484,6 +484,35 @@ __device__ __forceinline__ void fft_copy_top(int R, int C, int lgC, uint32_t* ld
   __syncthreads();
 }
 
+// acc = C(g) * xv: wave-uniform group indices take scalar branches; a
+// subfield constant (high half zero: every layer k >= m - 16) takes two
+// 16-bit products instead of the Karatsuba product.
+template <int NP>
+__device__ __forceinline__ void fft_mul(const uint32_t* ctab, uint32_t g, const uint32_t* xv, uint32_t* acc) {
+  const uint32_t g0 = __builtin_amdgcn_readfirstlane(g);
+  if constexpr (NP == 16) {
+    // one GF(2^16) half, twiddle in GF(2^16)
+    if (__all(g == g0))
+      gf16_mulc(xv, __builtin_amdgcn_readfirstlane(ctab_const(ctab, g0)), acc);
+    else
+      gf16_mulc(xv, ctab_const(ctab, g), acc);
+  } else {
+    if (__all(g == g0)) {
+      const uint32_t cst = __builtin_amdgcn_readfirstlane(ctab_const(ctab, g0));
+      if ((cst >> 16) == 0u)
+        gf32_mulc_sub<true>(xv, cst, acc);
+      else
+        gf32_mulc<true>(xv, cst, acc);
+    } else {
+      const uint32_t cst = ctab_const(ctab, g);
+      if (__all((cst >> 16) == 0u))
+        gf32_mulc_sub<false>(xv, cst, acc);
+      else
+        gf32_mulc<false>(xv, cst, acc);
+    }
+  }
+}
+
 template <int NP>
 __device__ __forceinline__ void fft_layer(int lo, int R, int C, int lgC, uint32_t* lds, int64_t hib, int64_t cb,
                                           int k, bool inv) {
@@ -515,30 +544,7 @@ __device__ __forceinline__ void fft_layer(int lo, int R, int C, int lgC, uint32_
 #pragma unroll
       for (int p = 0; p < NP; p++) xv[p] = pv[4 * p];
     }
-    const uint32_t g0 = __builtin_amdgcn_readfirstlane((uint32_t)g);
-    if constexpr (NP == 16) {
-      // one GF(2^16) half, twiddle in GF(2^16)
-      if (__all((uint32_t)g == g0))
-        gf16_mulc(xv, __builtin_amdgcn_readfirstlane(ctab_const(ctab, g0)), acc);
-      else
-        gf16_mulc(xv, ctab_const(ctab, (uint32_t)g), acc);
-    } else {
-      // the twiddle lies in the subfield GF(2^16) (high half zero) for every
-      // layer k >= m - 16: two 16-bit products instead of one 32-bit product
-      if (__all((uint32_t)g == g0)) {
-        const uint32_t cst = __builtin_amdgcn_readfirstlane(ctab_const(ctab, g0));
-        if ((cst >> 16) == 0u)
-          gf32_mulc_sub<true>(xv, cst, acc);
-        else
-          gf32_mulc<true>(xv, cst, acc);
-      } else {
-        const uint32_t cst = ctab_const(ctab, (uint32_t)g);
-        if (__all((cst >> 16) == 0u))
-          gf32_mulc_sub<false>(xv, cst, acc);
-        else
-          gf32_mulc<false>(xv, cst, acc);
-      }
-    }
+    fft_mul<NP>(ctab, (uint32_t)g, xv, acc);
     if (inv) {
 #pragma unroll
       for (int p = 0; p < NP; p++) pu[4 * p] ^= acc[p];

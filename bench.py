@@ -407,7 +407,7 @@ def small_code_leg(torch, cuot_amd, ctx, counters):
         del a_bits, ahat, scratch, x, y
     mf = next((v for k, v in (counters or {}).items() if k.startswith("qc_mf_encode")), None)
     return {"workload": "QC encode of one party, small n, both encoders on one prepared code",
-            "sizes": rows, "builtin_crossover_n": 43000,
+            "sizes": rows, "builtin_crossover_n": int(ctx.lib.cuot_qc_mfma_crossover()),
             "crossover_rule": "matrix cores for 17 <= n <= crossover (knob qc_mfma_max), additive FFT above",
             "roofline_mfma": {"bound": "mfma", "kernel": "qc_mf_encode", "unit": "TOP/s", "peak": MFMA_I8_PEAK_TOPS,
                               "achieved": rows[-1]["mfma_int8_TOPs"], "frac": rows[-1]["mfma_frac_of_dense_peak"],

@@ -126,6 +126,7 @@ SIGNATURES = {
     "cuot_mlp_adam": (INT, [P, P, P, P, P, INT, INT, C.c_float, P]),
     "cuot_set_knobs": (INT, [C.POINTER(KnobValues)]),
     "cuot_get_knobs": (INT, [C.POINTER(KnobValues)]),
+    "cuot_qc_mfma_crossover": (I64, []),
     "cuot_timing_begin": (INT, [P]),
     "cuot_timing_end": (INT, [P, C.POINTER(LaunchTime), INT, C.POINTER(INT)]),
     "cuot_ferret_session_new": (INT, [P, C.POINTER(FerretConfig), C.POINTER(P)]),

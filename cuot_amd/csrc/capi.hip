@@ -286,6 +286,8 @@ extern "C" int cuot_set_knobs(const cuot_knob_values* k) {
   return CUOT_OK;
 }
 
+extern "C" int64_t cuot_qc_mfma_crossover(void) { return cuot_qc_mfma_limit(); }
+
 extern "C" int cuot_get_knobs(cuot_knob_values* k) {
   CUOT_CHECK_ARG(k, "NULL knobs");
   k->qc_radix = g_knobs.qc_radix;

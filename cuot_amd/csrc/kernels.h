@@ -95,6 +95,7 @@ hipError_t cuot_launch_qc_encode_batch(const QcIo* io_d, const uint32_t* const* 
 // encoder can run on them; the encode picks by cuot_qc_mfma_selected.
 bool cuot_qc_mfma_eligible(int64_t n);
 bool cuot_qc_mfma_selected(int64_t n);  // eligible and at or below the crossover knob
+int64_t cuot_qc_mfma_limit();           // the crossover in force (knob or built-in)
 int64_t cuot_qc_ahat_words(int64_t n);
 hipError_t cuot_launch_qc_keep_bits(const QcPrep* pr, QcPrep pr1, int B, int m, int64_t nmax, hipStream_t st);
 hipError_t cuot_launch_qc_mfma(const QcIo* io, QcIo io1, const uint32_t* bits1, const uint32_t* const* ahats, int B,

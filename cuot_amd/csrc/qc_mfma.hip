@@ -13,9 +13,11 @@
 //
 // Operands.
 //   B: E is unpacked once per encode into bytes, E8[k/16][r][k%16]: the B
-//      operand of one lane (column r, 16 consecutive k) is one 16-byte load.
-//      Only ceil(n/32)*32 rows of k are stored (zero beyond n), which is at
-//      most the FFT's 64 planes x 2^m words, so the encode scratch is shared.
+//      operand of one lane (column r, 16 consecutive k) is one 16-byte word.
+//      The four waves of a workgroup share each k-step's 4 KB of E8 through
+//      LDS (three buffers, one barrier per step).  Only ceil(n/32)*32 rows of
+//      k are stored (zero beyond n), which is at most the FFT's 64 planes x
+//      2^m words, so the encode scratch is shared.
 //   A: the circulant never exists in memory.  A workgroup holds the bytes
 //      a_{(-j) mod n} for the window of j = k - i it touches in LDS; lane
 //      (row i, k-group g) reads its 16 bytes a_{(i-k) mod n}, k = k0+16g .. +15,
@@ -41,7 +43,7 @@ constexpr int64_t MF_NMAX = 1 << 16;          // bits of a are kept after the tr
 constexpr int64_t MF_TARGET_WGS = 1024;       // rows x slices aimed for (four per CU)
 constexpr int64_t MF_KLEN_MIN = 256, MF_KLEN_MAX = 16384;
 constexpr int MF_WIN_BYTES = MF_KLEN_MAX + MF_RW + 16;
-constexpr int64_t MF_DEFAULT_MAX = 43000;  // built-in crossover (n at or below: matrix cores), measured
+constexpr int64_t MF_DEFAULT_MAX = 49152;  // built-in crossover (n at or below: matrix cores), measured
 
 struct MfPlan {
   int64_t gx;    // row workgroups
@@ -104,6 +106,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void q
                                                     const uint4* e8_base, int64_t e8_stride) {
   __shared__ uint32_t win[MF_WIN_BYTES / 4];
   __shared__ uint4 ow[MF_WAVES][32 * MF_RT];  // parity words of each wave's rows
+  __shared__ mf_v4 bsh[3][256];               // B tiles of three k-steps
   const QcIo p = io ? io[blockIdx.z] : io1;
   const int64_t n = p.n;
   const MfPlan pl = mf_plan(n);
@@ -151,39 +154,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void q
 
   // the step count is wave-uniform (64-bit divisions above ran on the VALU)
   const int nsteps = __builtin_amdgcn_readfirstlane((int)((kh - kl) / 32));
-  const mf_v4* bp = e8 + (kl / 16 + g) * 128 + li;
-  // operands of step s + 1 load while step s multiplies, in two register sets
-  // (the last step reloads its own, so every access stays inside the slice)
-  mf_v4 An = mf_a(wp, sh), B0[4], B1[4];
+  // B: the workgroup's four waves share each k-step's 4 KB tile of E8
+  // (E8[kb..kb+1][0..127]) through three LDS buffers: during step s a wave
+  // multiplies with B(s) in registers, reads B(s+1) from LDS and stores the
+  // tile of step s+2, which thread t carries as word t in a register; buffer
+  // (s+2) % 3 was last read in step s-1, before that step's barrier
+  const mf_v4* tp = e8 + (kl / 16) * 128 + threadIdx.x;
+  auto tile = [&](int st) { return tp[(int64_t)(st < nsteps ? st : nsteps - 1) * 256]; };
+  bsh[0][threadIdx.x] = tile(0);
+  bsh[1][threadIdx.x] = tile(1);
+  mf_v4 r = tile(2);
+  __syncthreads();
+  mf_v4 B[4], Bn[4];
 #pragma unroll
-  for (int c = 0; c < 4; c++) B0[c] = bp[32 * c];
-  auto mma = [&](const mf_v4(&B)[4]) {
+  for (int c = 0; c < 4; c++) B[c] = bsh[0][g * 128 + 32 * c + li];
+  mf_v4 An = mf_a(wp, sh);
+  int rb = 1, wb = 2;  // buffers of steps s + 1 and s + 2
+  for (int s = 0; s < nsteps; s++) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) Bn[c] = bsh[rb][g * 128 + 32 * c + li];
+    A[0] = An;
+    An = mf_a(wp + 8 * (s + 1 < nsteps ? s + 1 : s), sh);
 #pragma unroll
     for (int rt = 0; rt < MF_RT; rt++)
 #pragma unroll
       for (int c = 0; c < 4; c++) acc[rt][c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[rt], B[c], acc[rt][c], 0, 0, 0);
 #pragma unroll
     for (int rt = MF_RT - 1; rt >= 1; rt--) A[rt] = A[rt - 1];
-  };
-  auto fetch = [&](mf_v4(&B)[4], int sn) {
+    bsh[wb][threadIdx.x] = r;
+    r = tile(s + 3);
 #pragma unroll
-    for (int c = 0; c < 4; c++) B[c] = bp[(int64_t)sn * 256 + 32 * c];
-    An = mf_a(wp + 8 * sn, sh);
-  };
-  int s = 0;
-  for (; s + 1 < nsteps; s += 2) {
-    A[0] = An;
-    fetch(B1, s + 1);
-    __builtin_amdgcn_sched_barrier(0);  // the loads go out before the products
-    mma(B0);
-    A[0] = An;
-    fetch(B0, s + 2 < nsteps ? s + 2 : s + 1);
-    __builtin_amdgcn_sched_barrier(0);
-    mma(B1);
-  }
-  if (s < nsteps) {
-    A[0] = An;
-    mma(B0);
+    for (int c = 0; c < 4; c++) B[c] = Bn[c];
+    rb = rb == 2 ? 0 : rb + 1;
+    wb = wb == 2 ? 0 : wb + 1;
+    __syncthreads();
   }
 
   // parities to rows: accumulator v of lane l holds row 8(v/4) + 4(l/32) + v%4
@@ -254,11 +258,13 @@ int64_t cuot_qc_ahat_words(int64_t n) {
   return ((int64_t)1 << m) + (cuot_qc_mfma_eligible(n) ? (n + 31) / 32 : 0);
 }
 
-bool cuot_qc_mfma_selected(int64_t n) {
+int64_t cuot_qc_mfma_limit() {
   const int k = cuot_knobs().qc_mfma_max;
   const int64_t lim = k == 0 ? MF_DEFAULT_MAX : (k < 0 ? 0 : k);
-  return cuot_qc_mfma_eligible(n) && n <= lim;
+  return lim < MF_NMAX ? lim : MF_NMAX;
 }
+
+bool cuot_qc_mfma_selected(int64_t n) { return cuot_qc_mfma_eligible(n) && n <= cuot_qc_mfma_limit(); }
 
 hipError_t cuot_launch_qc_keep_bits(const QcPrep* pr, QcPrep pr1, int B, int m, int64_t nmax, hipStream_t st) {
   const int64_t bytes = 4 * ((nmax + 31) / 32);

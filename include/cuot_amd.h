@@ -430,6 +430,9 @@ typedef struct {
 } cuot_knob_values;
 int cuot_set_knobs(const cuot_knob_values *k);
 int cuot_get_knobs(cuot_knob_values *k);
+/* the largest n the QC encoder currently runs on the matrix cores (knob
+ * qc_mfma_max, or the built-in crossover measured on MI355X when it is 0) */
+int64_t cuot_qc_mfma_crossover(void);
 
 /* ---- per-kernel timing ----------------------------------------------------------------
  * No counterpart in the reference.  Between cuot_timing_begin and

@@ -64,3 +64,32 @@ def test_ferret_state_size_refuses_malformed(lib):
     # the bootstrap must leave outputs: M < n
     assert lib.cuot_ferret_state_size(C.byref(_ferret(M, k, t, h))) == -1
     assert lib.cuot_ferret_state_size(None) == -1
+
+
+def test_knobs_refuse_bad_values_and_set_the_crossover(lib):
+    """Knob validation on the host: out-of-range values are refused with a
+    message and leave the knobs unchanged; the matrix-core crossover follows
+    qc_mfma_max (0: the built-in value, -1: never, capped at 2^16)."""
+    from cuot_amd import _lib, tune
+
+    try:
+        base = int(lib.cuot_qc_mfma_crossover())
+        assert 17 <= base <= 1 << 16
+        for bad in (dict(qc_mfma_max=-2), dict(aes_impl=3), dict(qc_radix=9), dict(qc_cols=3)):
+            k = _lib.KnobValues(**{f: 0 for f, _ in _lib.KnobValues._fields_})
+            for f, v in bad.items():
+                setattr(k, f, v)
+            assert lib.cuot_set_knobs(C.byref(k)) == 1
+            assert lib.cuot_last_error()
+            assert int(lib.cuot_qc_mfma_crossover()) == base
+        tune.set_knobs(qc_mfma_max=-1)
+        assert int(lib.cuot_qc_mfma_crossover()) == 0
+        tune.set_knobs(qc_mfma_max=30000)
+        assert int(lib.cuot_qc_mfma_crossover()) == 30000
+        tune.set_knobs(qc_mfma_max=1 << 20)
+        assert int(lib.cuot_qc_mfma_crossover()) == 1 << 16
+        tune.set_knobs(aes_impl=2)
+        assert tune.get_knobs()["aes_impl"] == 2
+    finally:
+        tune.set_knobs()
+    assert int(lib.cuot_qc_mfma_crossover()) == base

@@ -69,7 +69,7 @@ VALU_LANE_OPS_PER_S = 256 * 64 * 2.4e9  # one wave64 VALU instruction per CU per
 COUNTER_PASSES = (
     ["FETCH_SIZE", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES",
      "SQ_WAVE_CYCLES"],
-    ["WRITE_SIZE"],
+    ["WRITE_SIZE", "SQ_INSTS_VALU_MFMA_I8", "SQ_INSTS_VALU_MFMA_MOPS_I8", "SQ_VALU_MFMA_BUSY_CYCLES"],
 )
 CALIB_BYTES = 1 << 28  # the device copy that calibrates FETCH_SIZE / WRITE_SIZE
 
@@ -211,8 +211,9 @@ def collect_counters():
         if "copyBuffer" in e["name"] or "rocclr" in e["name"]:
             continue
         base, full = kernel_base(e["name"])
-        k = table.setdefault(full, {"base": base, "n0": 0, "n1": 0, "ns": 0.0, "fetch": 0.0, "write": 0.0,
-                                    "valu": 0.0, "lds": 0.0, "conflict": 0.0})
+        k = table.setdefault(full, {"base": base, "n0": 0, "n1": 0, "ns": 0.0, "ns1": 0.0, "fetch": 0.0, "write": 0.0,
+                                    "valu": 0.0, "lds": 0.0, "conflict": 0.0, "mfma": 0.0, "mops": 0.0,
+                                    "mbusy": 0.0})
         if pi == 0:
             k["n0"] += 1
             k["ns"] += e["ns"]
@@ -222,7 +223,11 @@ def collect_counters():
             k["conflict"] += e.get("SQ_LDS_BANK_CONFLICT", 0.0)
         else:
             k["n1"] += 1
+            k["ns1"] += e["ns"]
             k["write"] += e.get("WRITE_SIZE", 0.0)
+            k["mfma"] += e.get("SQ_INSTS_VALU_MFMA_I8", 0.0)
+            k["mops"] += e.get("SQ_INSTS_VALU_MFMA_MOPS_I8", 0.0)
+            k["mbusy"] += e.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0)
     out = {}
     for full, k in table.items():
         if not k["n0"]:
@@ -233,6 +238,13 @@ def collect_counters():
                "valu_util": round(k["valu"] / k["n0"] * 64 / (us * 1e-6 * VALU_LANE_OPS_PER_S), 3) if us > 0 else None,
                "lds_instr_per_launch": round(k["lds"] / k["n0"]),
                "lds_bank_conflict_cycles_per_launch": round(k["conflict"] / k["n0"])}
+        if k["n1"] and k["mfma"] > 0:
+            ops = k["mops"] / k["n1"] * 512  # the counter counts int8 multiply-add ops / 512
+            us1 = k["ns1"] / k["n1"] / 1e3
+            row["mfma_i8_instr_per_launch"] = round(k["mfma"] / k["n1"])
+            row["mfma_i8_ops_per_launch"] = round(ops)
+            row["mfma_busy_cycles_per_launch"] = round(k["mbusy"] / k["n1"])
+            row["mfma_i8_TOPs_profiled"] = round(ops / (us1 * 1e-6) / 1e12, 1) if us1 > 0 else None
         if fscale and wscale and k["n1"]:
             tb = k["fetch"] / k["n0"] * 1024 * fscale + k["write"] / k["n1"] * 1024 * wscale
             row["hbm_bytes_per_launch"] = round(tb)
@@ -346,6 +358,34 @@ def cpu_baseline(threads: int):
     }
 
 
+QC_ENCODE_KERNELS = ("qc_pack_x", "transpose_fwd", "qc_conv_rows", "transpose_inv", "qc_fft_pass", "qc_unpack_x",
+                     "qc_rows_fold", "qc_output_planes", "qc_output")
+
+
+def qc_encode_bound(counters, qc_ms):
+    """Per-kernel bounds of one QC encode at n = 2^24 (the counter run's
+    encode): each kernel's time at full VALU issue and at the HBM peak, the
+    larger of the two, summed over its launches; against the profiled and the
+    live per-party times."""
+    if not counters:
+        return None
+    bound = prof = 0.0
+    parts = {}
+    for name, v in counters.items():
+        if kernel_base(name)[0] not in QC_ENCODE_KERNELS or "hbm_bytes_per_launch" not in v:
+            continue
+        valu_ms = v["valu_instr_per_launch"] * 64 / VALU_LANE_OPS_PER_S * 1e3
+        hbm_ms = v["hbm_bytes_per_launch"] / (HBM_PEAK_GBS * 1e9) * 1e3
+        b = max(valu_ms, hbm_ms) * v["launches"]
+        bound += b
+        prof += v["us_per_launch"] * v["launches"] / 1e3
+        parts[name] = {"bound_ms": round(b, 4), "by": "valu" if valu_ms > hbm_ms else "hbm",
+                       "profiled_ms": round(v["us_per_launch"] * v["launches"] / 1e3, 4)}
+    return {"sum_of_kernel_bounds_ms": round(bound, 4), "profiled_ms": round(prof, 4),
+            "live_ms_per_party": round(qc_ms, 4), "bound_over_live": round(bound / qc_ms, 4) if qc_ms else None,
+            "kernels": parts}
+
+
 def timed_kernels(cuot_amd, ctx, fn, reps=3):
     """Minimum over `reps` calls of fn of the summed per-launch times (HIP
     events around every launch), with the per-kernel split of that call."""
@@ -412,7 +452,13 @@ def small_code_leg(torch, cuot_amd, ctx, counters):
             "roofline_mfma": {"bound": "mfma", "kernel": "qc_mf_encode", "unit": "TOP/s", "peak": MFMA_I8_PEAK_TOPS,
                               "achieved": rows[-1]["mfma_int8_TOPs"], "frac": rows[-1]["mfma_frac_of_dense_peak"],
                               "n": rows[-1]["n"]},
-            "counters_n32768": mf}
+            "counters_n32768": mf,
+            "mfma_counter_check": None if not mf or "mfma_i8_ops_per_launch" not in mf else {
+                "n": 32768, "algorithmic_int8_ops": 2 * 128 * 32768 * 32768,
+                "counted_int8_ops": mf["mfma_i8_ops_per_launch"],
+                "counted_TOPs_at_profiled_time": mf.get("mfma_i8_TOPs_profiled"),
+                "frac_of_dense_peak": round(mf["mfma_i8_TOPs_profiled"] / MFMA_I8_PEAK_TOPS, 4)
+                if mf.get("mfma_i8_TOPs_profiled") else None}}
 
 
 def aes_leg(torch, cuot_amd, ctx, counters):
@@ -1011,7 +1057,8 @@ def main():
         "counters": {"note": counters_note, "kernels": counters},
         "kernels": per_kernel,
         "qc_encode": {"ms_per_party": round(qc_ms, 4), "compulsory_bytes": qc_bytes,
-                      "compulsory_GBps": round(qc_bytes / (qc_ms / 1e3) / 1e9, 1)},
+                      "compulsory_GBps": round(qc_bytes / (qc_ms / 1e3) / 1e9, 1),
+                      "bounds": qc_encode_bound(counters, qc_ms)},
         "ggm": {"aes_per_launch": aes_per_party, "ms_per_launch": round(ggm_ms, 4),
                 "aes_per_s": round(aes_per_party / (ggm_ms / 1e3), 1)},
         "qc_small_codes": small,

@@ -473,7 +473,8 @@ def aes_leg(torch, cuot_amd, ctx, counters):
     y = torch.empty_like(x)
     key = (0x0123456789ABCDEF, 0x0FEDCBA987654321)
     out, outs = {}, {}
-    for name, impl, kname in (("ttable", 1, "aes_stream_kernel<false>"), ("bitsliced", 2, "aes_bs_stream_kernel<false>")):
+    ciphers = (("ttable", 1, "aes_stream_kernel<false>"), ("bitsliced", 2, "aes_bs_stream_kernel<false>"))
+    for name, impl, kname in ciphers:
         tune.set_knobs(aes_impl=impl)
         ctx.aes_ecb(key, x, y)
         torch.cuda.synchronize()
@@ -830,7 +831,8 @@ def main():
     nl = sum(v["launches"] for v in variants)
     dom_ctr = None
     if nl:
-        dom_ctr = {"valu_instr_per_launch": round(sum(v["valu_instr_per_launch"] * v["launches"] for v in variants) / nl)}
+        valu = sum(v["valu_instr_per_launch"] * v["launches"] for v in variants)
+        dom_ctr = {"valu_instr_per_launch": round(valu / nl)}
         if all("hbm_bytes_per_launch" in v for v in variants):
             dom_ctr["hbm_bytes_per_launch"] = round(sum(v["hbm_bytes_per_launch"] * v["launches"] for v in variants)
                                                     / nl)

@@ -101,9 +101,9 @@ __device__ __forceinline__ mf_v4 mf_a(const uint32_t* p, uint32_t sh) {
 }
 
 // grid (row workgroups, slices, instances); 4 waves of 2 x 32 rows each.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void qc_mf_encode(const QcIo* io, QcIo io1, const uint32_t* bits1,
-                                                    const uint32_t* const* ahats, int64_t bits_off,
-                                                    const uint4* e8_base, int64_t e8_stride) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void qc_mf_encode(
+    const QcIo* io, QcIo io1, const uint32_t* bits1, const uint32_t* const* ahats, int64_t bits_off,
+    const uint4* e8_base, int64_t e8_stride) {
   __shared__ uint32_t win[MF_WIN_BYTES / 4];
   __shared__ uint4 ow[MF_WAVES][32 * MF_RT];  // parity words of each wave's rows
   __shared__ mf_v4 bsh[3][256];               // B tiles of three k-steps

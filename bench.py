@@ -490,6 +490,29 @@ def aes_leg(torch, cuot_amd, ctx, counters):
     out["faster"] = max(("ttable", "bitsliced"), key=lambda k: out[k]["aes_per_s"])
     out["library_rule"] = "ttable"
     out["blocks"] = n
+    del x, y
+    # the GGM sender of the SOT workload (t = 1024 trees of depth 15) with each cipher
+    t, h = 1024, 15
+    B = cuot_amd.to_block
+    p = cuot_amd.GgmParams(t=t, h=h, mode=0, n_out=t << h, k0=B(SOT_K0), k1=B(SOT_K1))
+    roots = torch.randint(-2**62, 2**62, (t, 2), dtype=torch.int64, generator=g).cuda()
+    leaves = [torch.empty((t << h, 2), dtype=torch.int64, device="cuda") for _ in range(2)]
+    sums = [torch.empty((t * h * 2, 2), dtype=torch.int64, device="cuda") for _ in range(2)]
+    work = torch.empty(ctx.ggm_sender_bs_work_bytes(p), dtype=torch.uint8, device="cuda")
+    calls = {"ttable": lambda: ctx.ggm_sender(p, roots, leaves[0], sums[0]),
+             "bitsliced": lambda: ctx.ggm_sender_bs(p, roots, leaves[1], sums[1], work)}
+    aes = 2 * t * ((1 << h) - 1)
+    ggm = {"config": {"t": t, "h": h, "aes": aes}}
+    for name, fn in calls.items():
+        fn()
+        torch.cuda.synchronize()
+        ms, _ = timed_kernels(cuot_amd, ctx, fn)
+        ggm[name] = {"ms": round(ms, 4), "aes_per_s": round(aes / (ms * 1e-3), 1)}
+    ggm["same_output"] = bool(torch.equal(leaves[0], leaves[1]) and torch.equal(sums[0], sums[1]))
+    ggm["form"] = ("ttable: one fused launch (breadth-first top in LDS, per-lane depth-first subtrees); "
+                   "bitsliced: one launch per level through HBM, 32 nodes per lane")
+    out["ggm_sender"] = ggm
+    del leaves, sums, work, roots
     return out
 
 

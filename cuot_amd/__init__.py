@@ -81,6 +81,15 @@ class Context:
         check(self.lib.cuot_ggm_sender(self.handle, C.byref(params), _ptr(roots), _ptr(leaves), _ptr(sums),
                                        _stream_ptr(stream)), "cuot_ggm_sender")
 
+    def ggm_sender_bs_work_bytes(self, params: GgmParams) -> int:
+        return int(self.lib.cuot_ggm_sender_bs_work_bytes(C.byref(params)))
+
+    def ggm_sender_bs(self, params: GgmParams, roots, leaves, sums, work, stream=None):
+        """The sender expansion on the bitsliced AES (breadth-first, one launch per level)."""
+        check(self.lib.cuot_ggm_sender_bs(self.handle, C.byref(params), _ptr(roots), _ptr(leaves), _ptr(sums),
+                                          _ptr(work), work.numel() * work.element_size(), _stream_ptr(stream)),
+              "cuot_ggm_sender_bs")
+
     def ggm_recver(self, params: GgmParams, choice, csum, fin, leaves, stream=None):
         check(self.lib.cuot_ggm_recver(self.handle, C.byref(params), _ptr(choice), _ptr(csum), _ptr(fin),
                                        _ptr(leaves), _stream_ptr(stream)), "cuot_ggm_recver")

@@ -1,5 +1,5 @@
-// aes_bs.hip -- AES-128 ECB / counter-mode streams with the bitsliced cipher
-// (aes_bs.h): no tables and no LDS.  One lane encrypts 32 blocks; a wave
+// aes_bs.hip -- AES-128 ECB / counter-mode streams and a breadth-first GGM
+// sender with the bitsliced cipher (aes_bs.h): no tables and no LDS.  One lane encrypts 32 blocks; a wave
 // covers 2048 consecutive blocks, lane l taking blocks 64 k + l (k = 0..31),
 // so every load and store instruction of the wave touches 1 KiB of
 // consecutive memory.  The T-table kernels (aes.hip) and these compute the
@@ -62,7 +62,111 @@ hipError_t launch(BsArgs& a, const uint32_t rk[44], bool prg, hipStream_t st) {
   return hipGetLastError();
 }
 
+// ---- GGM sender on the bitsliced cipher ------------------------------------
+// Breadth-first, one launch per level: lane g takes the 32 consecutive
+// parents 32g .. 32g+31 of the level, encrypts them under k0 and under k1
+// (32 blocks per lane, bit-sliced), adds the parent (child = AES_k(p) ^ p) and
+// writes children 2j (k0) and 2j+1 (k1).  The level sums of every tree are
+// XORed into sums[tree][d][side] with 64-bit atomics: per block for levels
+// with fewer than 32 nodes per tree, else reduced over the lanes of one tree
+// first (2^(d-5) lanes, aligned groups of a wave).
+struct GgmBsArgs {
+  uint32_t rk[2][44];  // k0, k1 round keys with the S-box constant folded in
+  const blk* par;      // t * 2^d parents
+  blk* out;            // children, tree-major (the leaves at the last level)
+  blk* sums;           // [t][h][2]
+  int64_t np;          // parents
+  int64_t n_out;       // children with index >= n_out are not written
+  int d, h;
+  int clear_lsb;       // last level in Ferret mode: leaves with bit 0 cleared
+};
+
+CUOT_DEV void sums_xor(blk* s, blk v) {
+  unsigned long long* q = (unsigned long long*)s;
+  atomicXor(q, (unsigned long long)v.x | ((unsigned long long)v.y << 32));
+  atomicXor(q + 1, (unsigned long long)v.z | ((unsigned long long)v.w << 32));
+}
+
+__global__ __launch_bounds__(256) void ggm_bs_level(GgmBsArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t j0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 32;
+  for (int side = 0; side < 2; side++) {
+    uint32_t S[128];
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+      const blk p = j0 + k < a.np ? a.par[j0 + k] : bzero();
+      S[k] = p.x;
+      S[32 + k] = p.y;
+      S[64 + k] = p.z;
+      S[96 + k] = p.w;
+    }
+#pragma unroll
+    for (int w = 0; w < 4; w++) bs_transpose32(S + 32 * w);
+    bs_encrypt(S, a.rk[side]);
+#pragma unroll
+    for (int w = 0; w < 4; w++) bs_transpose32(S + 32 * w);
+    blk acc = bzero();
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+      const int64_t j = j0 + k;
+      if (j >= a.np) continue;
+      const blk c = bxor(make_uint4(S[k], S[32 + k], S[64 + k], S[96 + k]), a.par[j]);
+      const int64_t ci = 2 * j + side;
+      if (ci < a.n_out) stb(a.out + ci, a.clear_lsb ? make_uint4(c.x & ~1u, c.y, c.z, c.w) : c);
+      if (a.d < 5)
+        sums_xor(a.sums + ((j >> a.d) * a.h + a.d) * 2 + side, c);
+      else
+        acc = bxor(acc, c);
+    }
+    if (a.d >= 5) {
+      // lanes of one tree: 2^(d-5), capped at the wave
+      const int G = a.d - 5 >= 6 ? 64 : 1 << (a.d - 5);
+      for (int o = G >> 1; o >= 1; o >>= 1) {
+        acc.x ^= (uint32_t)__shfl_xor((int)acc.x, o, 64);
+        acc.y ^= (uint32_t)__shfl_xor((int)acc.y, o, 64);
+        acc.z ^= (uint32_t)__shfl_xor((int)acc.z, o, 64);
+        acc.w ^= (uint32_t)__shfl_xor((int)acc.w, o, 64);
+      }
+      const int64_t tree = j0 >> a.d;
+      if ((lane & (G - 1)) == 0 && j0 < a.np) sums_xor(a.sums + (tree * a.h + a.d) * 2 + side, acc);
+    }
+  }
+}
+
+__global__ void ggm_bs_zero(blk* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = bzero();
+}
+
 }  // namespace
+
+size_t cuot_ggm_bs_work_bytes(int64_t t, int h) { return 2 * (size_t)(t << (h - 1)) * 16; }
+
+hipError_t cuot_launch_ggm_sender_bs(const uint32_t rk[88], int64_t t, int h, int ferret, int64_t n_out,
+                                     const blk* roots, blk* leaves, blk* sums, void* work, hipStream_t st) {
+  GgmBsArgs a;
+  bs_fold_sbox_constant(rk, a.rk[0]);
+  bs_fold_sbox_constant(rk + 44, a.rk[1]);
+  a.sums = sums;
+  a.h = h;
+  const int64_t nsums = t * h * 2;
+  hipLaunchKernelGGL(ggm_bs_zero, dim3((unsigned)((nsums + 255) / 256 < 4096 ? (nsums + 255) / 256 : 4096)),
+                     dim3(256), 0, st, sums, nsums);
+  blk* buf[2] = {(blk*)work, (blk*)work + (t << (h - 1))};
+  for (int d = 0; d < h; d++) {
+    a.d = d;
+    a.np = t << d;
+    a.par = d == 0 ? roots : buf[d & 1];
+    const bool last = d == h - 1;
+    a.out = last ? leaves : buf[(d + 1) & 1];
+    a.n_out = last ? n_out : 2 * a.np;
+    a.clear_lsb = last && ferret;
+    const int64_t lanes = (a.np + 31) / 32;
+    cuot_rec_begin(st, "ggm_bs_level", 16.0 * 3.0 * (double)a.np);
+    hipLaunchKernelGGL(ggm_bs_level, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, a);
+    cuot_rec_end(st);
+  }
+  return hipGetLastError();
+}
 
 hipError_t cuot_launch_aes_ecb_bs(const uint32_t rk[44], const blk* in, blk* out, int64_t n, hipStream_t st) {
   if (n <= 0) return hipSuccess;

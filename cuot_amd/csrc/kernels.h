@@ -38,6 +38,11 @@ hipError_t cuot_launch_prg(const uint32_t rk[44], uint64_t stream, uint64_t star
 hipError_t cuot_launch_aes_ecb_bs(const uint32_t rk[44], const blk* in, blk* out, int64_t n, hipStream_t st);
 hipError_t cuot_launch_prg_bs(const uint32_t rk[44], uint64_t stream, uint64_t start, int64_t n, blk* out,
                               hipStream_t st);
+// GGM sender (full trees, level sums) breadth-first on the bitsliced cipher;
+// rk: k0 then k1 round keys; work: cuot_ggm_bs_work_bytes bytes
+size_t cuot_ggm_bs_work_bytes(int64_t t, int h);
+hipError_t cuot_launch_ggm_sender_bs(const uint32_t rk[88], int64_t t, int h, int ferret, int64_t n_out,
+                                     const blk* roots, blk* leaves, blk* sums, void* work, hipStream_t st);
 
 // Host helpers (aes_host.cpp).
 void cuot_aes_expand_host(const uint8_t key[16], uint32_t rk[44]);

@@ -79,6 +79,15 @@ typedef struct {
 /* sums_d: [t][h][2] = xor of even / odd nodes at depth d+1. */
 int cuot_ggm_sender(cuot_ctx *ctx, const cuot_ggm_params *p, const cuot_block *roots_d, cuot_block *leaves_d,
                     cuot_block *sums_d, void *stream);
+/* The same sender outputs from the bitsliced AES variant, breadth-first with
+ * one launch per level through a caller work buffer of
+ * cuot_ggm_sender_bs_work_bytes(p) bytes (two levels of t * 2^(h-1) blocks);
+ * the LDS T-table form above is the one the library's pipelines use.  Like
+ * cuot_ggm_sender it replaces the per-level Aes::expand + Mat::sum of
+ * gpu/aes_expand.cu:38-46, gpu/gpu_matrix.cu:155-169. */
+size_t cuot_ggm_sender_bs_work_bytes(const cuot_ggm_params *p);
+int cuot_ggm_sender_bs(cuot_ctx *ctx, const cuot_ggm_params *p, const cuot_block *roots_d, cuot_block *leaves_d,
+                       cuot_block *sums_d, void *work_d, size_t work_bytes, void *stream);
 /* choice_d: [t][h] bytes (1 = receiver chose the odd side at that level; the
  * punctured path follows the other side).  csum_d: [t][h] chosen sums.
  * fin_d: [t]: SOT = final-row value (m_c[h] ^ mc[h-1]); Ferret = secret sum. */

@@ -153,3 +153,30 @@ def test_ggm_sender_receiver(ctx, h, t, mode):
     if mode == 1:
         ow = po.ferret_mpcot_recver(t, h, c.reshape(-1), csum, ss, n_out)
         assert np.array_equal(w, ow)
+
+
+# the sender on the bitsliced cipher (breadth-first, one launch per level)
+# equals the LDS T-table sender: leaves, truncation, Ferret LSB rule and the
+# level sums of every tree, for levels with fewer and more than 32 nodes per
+# tree and for trees sharing and spanning waves
+@pytest.mark.parametrize("h,t", [(1, 3), (3, 2), (5, 7), (6, 64), (9, 5), (11, 3), (13, 2), (15, 2)])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_ggm_sender_bitsliced_matches_ttable(ctx, h, t, mode):
+    import torch
+
+    print(f"ggm bitsliced h={h} t={t} mode={mode}", flush=True)
+    r = rng(3000 + 17 * h + t + mode)
+    k0, k1 = (rand_blocks(r, 1)[0], rand_blocks(r, 1)[0]) if mode == 0 else (FERRET_K0, FERRET_K1)
+    roots = rand_blocks(r, t)
+    n_out = (t << h) - (5 if h >= 4 else 0)
+    p = _params(t, h, mode, n_out, k0, k1)
+    a_leaves, a_sums = dev_blocks(n_out), dev_blocks(t * h * 2)
+    ctx.ggm_sender(p, to_dev(roots), a_leaves, a_sums)
+    b_leaves, b_sums = dev_blocks(n_out), dev_blocks(t * h * 2)
+    work = torch.empty(ctx.ggm_sender_bs_work_bytes(p), dtype=torch.uint8, device="cuda")
+    ctx.ggm_sender_bs(p, to_dev(roots), b_leaves, b_sums, work)
+    assert np.array_equal(to_np(b_leaves)[:n_out], to_np(a_leaves)[:n_out])
+    assert np.array_equal(to_np(b_sums), to_np(a_sums))
+    if mode == 0 and t <= 3:
+        lv, l, rr = po.ggm_sender_tree(k0, k1, roots[0], h)
+        assert np.array_equal(to_np(b_leaves)[: 1 << h], lv)

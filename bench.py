@@ -512,7 +512,22 @@ def aes_leg(torch, cuot_amd, ctx, counters):
     ggm["form"] = ("ttable: one fused launch (breadth-first top in LDS, per-lane depth-first subtrees); "
                    "bitsliced: one launch per level through HBM, 32 nodes per lane")
     out["ggm_sender"] = ggm
-    del leaves, sums, work, roots
+    # the receiver of the same trees (one punctured path per tree) with each cipher
+    choice = torch.randint(0, 2, (t * h,), dtype=torch.uint8, generator=g).cuda()
+    csum = sums[0].view(t * h, 2, 2)[torch.arange(t * h, device="cuda"), choice.long()].contiguous()
+    fin = torch.randint(-2**62, 2**62, (t, 2), dtype=torch.int64, generator=g).cuda()
+    rwork = torch.empty(ctx.ggm_recver_bs_work_bytes(p), dtype=torch.uint8, device="cuda")
+    rcalls = {"ttable": lambda: ctx.ggm_recver(p, choice, csum, fin, leaves[0]),
+              "bitsliced": lambda: ctx.ggm_recver_bs(p, choice, csum, fin, leaves[1], rwork)}
+    rec = {"config": ggm["config"]}
+    for name, fn in rcalls.items():
+        fn()
+        torch.cuda.synchronize()
+        ms, _ = timed_kernels(cuot_amd, ctx, fn)
+        rec[name] = {"ms": round(ms, 4), "aes_per_s": round(aes / (ms * 1e-3), 1)}
+    rec["same_output"] = bool(torch.equal(leaves[0], leaves[1]))
+    out["ggm_recver"] = rec
+    del leaves, sums, work, rwork, roots, choice, csum, fin
     return out
 
 

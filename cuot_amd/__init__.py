@@ -90,6 +90,15 @@ class Context:
                                           _ptr(work), work.numel() * work.element_size(), _stream_ptr(stream)),
               "cuot_ggm_sender_bs")
 
+    def ggm_recver_bs_work_bytes(self, params: GgmParams) -> int:
+        return int(self.lib.cuot_ggm_recver_bs_work_bytes(C.byref(params)))
+
+    def ggm_recver_bs(self, params: GgmParams, choice, csum, fin, leaves, work, stream=None):
+        """The receiver expansion on the bitsliced AES (breadth-first, per-level sibling fix)."""
+        check(self.lib.cuot_ggm_recver_bs(self.handle, C.byref(params), _ptr(choice), _ptr(csum), _ptr(fin),
+                                          _ptr(leaves), _ptr(work), work.numel() * work.element_size(),
+                                          _stream_ptr(stream)), "cuot_ggm_recver_bs")
+
     def ggm_recver(self, params: GgmParams, choice, csum, fin, leaves, stream=None):
         check(self.lib.cuot_ggm_recver(self.handle, C.byref(params), _ptr(choice), _ptr(csum), _ptr(fin),
                                        _ptr(leaves), _stream_ptr(stream)), "cuot_ggm_recver")

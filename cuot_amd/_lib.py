@@ -81,6 +81,8 @@ SIGNATURES = {
     "cuot_ggm_sender": (INT, [P, C.POINTER(GgmParams), P, P, P, P]),
     "cuot_ggm_sender_bs_work_bytes": (C.c_size_t, [C.POINTER(GgmParams)]),
     "cuot_ggm_sender_bs": (INT, [P, C.POINTER(GgmParams), P, P, P, P, C.c_size_t, P]),
+    "cuot_ggm_recver_bs_work_bytes": (C.c_size_t, [C.POINTER(GgmParams)]),
+    "cuot_ggm_recver_bs": (INT, [P, C.POINTER(GgmParams), P, P, P, P, P, C.c_size_t, P]),
     "cuot_ggm_recver": (INT, [P, C.POINTER(GgmParams), P, P, P, P, P]),
     "cuot_lpn": (INT, [P, C.POINTER(LpnParams), P, P, P, P]),
     "cuot_lpn_dual": (INT, [P, C.POINTER(LpnParams), P, P, P, P, P, P, P]),

@@ -4,6 +4,8 @@
 // so every load and store instruction of the wave touches 1 KiB of
 // consecutive memory.  The T-table kernels (aes.hip) and these compute the
 // same function; the knob aes_impl picks one (execution choice only).
+#include <string.h>
+
 #include "aes_bs.h"
 #include "kernels.h"
 
@@ -137,9 +139,115 @@ __global__ void ggm_bs_zero(blk* p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = bzero();
 }
 
+// Receiver, after level d's expansion of its own nodes (the punctured node
+// holds 0, so its children are AES_k0(0), AES_k1(0), taken out of the sums):
+// the chosen sum gives the sibling of the punctured path, the other child
+// stays unknown (0) and becomes the punctured node of the next level; at the
+// last level it takes the final rule (SOT: fin ^ opposite-side sum, Ferret:
+// fin ^ the LSB-cleared xor of the other leaves).  One lane per tree.
+struct GgmBsFix {
+  const blk* sums;  // [t][h][2] of the receiver's nodes
+  int64_t* punct;   // punctured index per tree at level d
+  const uint8_t* choice;
+  const blk* csum;
+  const blk* fin;
+  blk* out;       // level d+1 nodes, or the leaves
+  int64_t n_out;  // leaves written below this index (last level)
+  int64_t t;
+  int d, h, last, ferret;
+  blk z0, z1;  // children of a zero parent
+};
+
+__global__ void ggm_bs_fix(GgmBsFix a) {
+  const int64_t tree = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (tree >= a.t) return;
+  const int64_t p = a.punct[tree];
+  const blk L = bxor(a.sums[(tree * a.h + a.d) * 2], a.z0);
+  const blk R = bxor(a.sums[(tree * a.h + a.d) * 2 + 1], a.z1);
+  const int c = a.choice[tree * a.h + a.d] ? 1 : 0;
+  const int64_t sib = 2 * p + c, np = 2 * p + 1 - c;
+  const blk sibv = bxor(a.csum[tree * a.h + a.d], c ? R : L);
+  const int64_t base = tree << (a.d + 1);
+  if (!a.last) {
+    a.out[base + sib] = sibv;
+    a.out[base + np] = bzero();
+    a.punct[tree] = np;
+    return;
+  }
+  blk v;
+  if (!a.ferret) {
+    v = bxor(a.fin[tree], c ? L : R);
+  } else {
+    blk o = bxor(bxor(L, R), sibv);
+    o.x &= ~1u;
+    v = bxor(a.fin[tree], o);
+  }
+  if (base + sib < a.n_out) a.out[base + sib] = a.ferret ? make_uint4(sibv.x & ~1u, sibv.y, sibv.z, sibv.w) : sibv;
+  if (base + np < a.n_out) a.out[base + np] = v;
+}
+
+__global__ void ggm_bs_zero64(int64_t* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0;
+}
+
+unsigned small_grid(int64_t n) { return (unsigned)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096); }
+
 }  // namespace
 
 size_t cuot_ggm_bs_work_bytes(int64_t t, int h) { return 2 * (size_t)(t << (h - 1)) * 16; }
+
+size_t cuot_ggm_recver_bs_work_bytes(int64_t t, int h) {
+  return cuot_ggm_bs_work_bytes(t, h) + (size_t)t * 16 + (size_t)t * h * 2 * 16 + (size_t)t * 8;
+}
+
+hipError_t cuot_launch_ggm_recver_bs(const uint32_t rk[88], int64_t t, int h, int ferret, int64_t n_out,
+                                     const uint8_t* choice, const blk* csum, const blk* fin, blk* leaves, void* work,
+                                     hipStream_t st) {
+  GgmBsArgs a;
+  bs_fold_sbox_constant(rk, a.rk[0]);
+  bs_fold_sbox_constant(rk + 44, a.rk[1]);
+  blk* buf[2] = {(blk*)work, (blk*)work + (t << (h - 1))};
+  blk* zroots = buf[1] + (t << (h - 1));
+  blk* sums = zroots + t;
+  int64_t* punct = (int64_t*)(sums + t * h * 2);
+  hipLaunchKernelGGL(ggm_bs_zero, dim3(small_grid(t + t * h * 2)), dim3(256), 0, st, zroots, t + t * h * 2);
+  hipLaunchKernelGGL(ggm_bs_zero64, dim3(small_grid(t)), dim3(256), 0, st, punct, t);
+  GgmBsFix f;
+  f.sums = sums;
+  f.punct = punct;
+  f.choice = choice;
+  f.csum = csum;
+  f.fin = fin;
+  f.t = t;
+  f.h = h;
+  f.ferret = ferret;
+  uint8_t zero[16] = {0}, e0[16], e1[16];
+  cuot_aes_encrypt_host(rk, zero, e0);
+  cuot_aes_encrypt_host(rk + 44, zero, e1);
+  memcpy(&f.z0, e0, 16);
+  memcpy(&f.z1, e1, 16);
+  a.sums = sums;
+  a.h = h;
+  for (int d = 0; d < h; d++) {
+    a.d = d;
+    a.np = t << d;
+    a.par = d == 0 ? zroots : buf[d & 1];
+    const bool last = d == h - 1;
+    a.out = last ? leaves : buf[(d + 1) & 1];
+    a.n_out = last ? n_out : 2 * a.np;
+    a.clear_lsb = last && ferret;
+    const int64_t lanes = (a.np + 31) / 32;
+    cuot_rec_begin(st, "ggm_bs_level", 16.0 * 3.0 * (double)a.np);
+    hipLaunchKernelGGL(ggm_bs_level, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, a);
+    cuot_rec_end(st);
+    f.d = d;
+    f.last = last;
+    f.out = a.out;
+    f.n_out = last ? n_out : 2 * a.np;
+    hipLaunchKernelGGL(ggm_bs_fix, dim3(small_grid(t)), dim3(256), 0, st, f);
+  }
+  return hipGetLastError();
+}
 
 hipError_t cuot_launch_ggm_sender_bs(const uint32_t rk[88], int64_t t, int h, int ferret, int64_t n_out,
                                      const blk* roots, blk* leaves, blk* sums, void* work, hipStream_t st) {

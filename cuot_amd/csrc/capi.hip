@@ -144,6 +144,28 @@ extern "C" int cuot_ggm_sender_bs(cuot_ctx* ctx, const cuot_ggm_params* p, const
   return CUOT_OK;
 }
 
+extern "C" size_t cuot_ggm_recver_bs_work_bytes(const cuot_ggm_params* p) {
+  if (ggm_check(p)) return 0;
+  return cuot_ggm_recver_bs_work_bytes(p->t, p->h);
+}
+
+extern "C" int cuot_ggm_recver_bs(cuot_ctx* ctx, const cuot_ggm_params* p, const uint8_t* choice_d,
+                                  const cuot_block* csum_d, const cuot_block* fin_d, cuot_block* leaves_d,
+                                  void* work_d, size_t work_bytes, void* stream) {
+  CUOT_CHECK_ARG(ctx, "ctx is NULL");
+  int rc = ggm_check(p);
+  if (rc) return rc;
+  CUOT_CHECK_ARG(choice_d && csum_d && fin_d && work_d && (leaves_d || p->n_out == 0), "buffer is NULL");
+  CUOT_CHECK_ARG(work_bytes >= cuot_ggm_recver_bs_work_bytes(p->t, p->h), "work buffer too small");
+  CUOT_HIP_TRY(hipSetDevice(ctx->device));
+  GgmKArgs a;
+  ggm_fill(a, p);
+  CUOT_HIP_TRY(cuot_launch_ggm_recver_bs(a.rk, a.t, a.h, a.mode == CUOT_MODE_FERRET, a.n_out, choice_d,
+                                         (const blk*)csum_d, (const blk*)fin_d, (blk*)leaves_d, work_d,
+                                         ctx->st(stream)));
+  return CUOT_OK;
+}
+
 extern "C" int cuot_ggm_recver(cuot_ctx* ctx, const cuot_ggm_params* p, const uint8_t* choice_d,
                                const cuot_block* csum_d, const cuot_block* fin_d, cuot_block* leaves_d,
                                void* stream) {

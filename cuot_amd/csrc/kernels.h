@@ -43,6 +43,11 @@ hipError_t cuot_launch_prg_bs(const uint32_t rk[44], uint64_t stream, uint64_t s
 size_t cuot_ggm_bs_work_bytes(int64_t t, int h);
 hipError_t cuot_launch_ggm_sender_bs(const uint32_t rk[88], int64_t t, int h, int ferret, int64_t n_out,
                                      const blk* roots, blk* leaves, blk* sums, void* work, hipStream_t st);
+// receiver from its own view (choice bits, chosen sums, final blocks)
+size_t cuot_ggm_recver_bs_work_bytes(int64_t t, int h);
+hipError_t cuot_launch_ggm_recver_bs(const uint32_t rk[88], int64_t t, int h, int ferret, int64_t n_out,
+                                     const uint8_t* choice, const blk* csum, const blk* fin, blk* leaves, void* work,
+                                     hipStream_t st);
 
 // Host helpers (aes_host.cpp).
 void cuot_aes_expand_host(const uint8_t key[16], uint32_t rk[44]);

@@ -93,6 +93,14 @@ int cuot_ggm_sender_bs(cuot_ctx *ctx, const cuot_ggm_params *p, const cuot_block
  * fin_d: [t]: SOT = final-row value (m_c[h] ^ mc[h-1]); Ferret = secret sum. */
 int cuot_ggm_recver(cuot_ctx *ctx, const cuot_ggm_params *p, const uint8_t *choice_d, const cuot_block *csum_d,
                     const cuot_block *fin_d, cuot_block *leaves_d, void *stream);
+/* The same receiver outputs from the bitsliced AES variant (breadth-first,
+ * one expansion and one sibling-fix launch per level) through a caller work
+ * buffer of cuot_ggm_recver_bs_work_bytes(p) bytes; for the cipher comparison,
+ * the pipelines use cuot_ggm_recver.  Replaces the receiver level loop of
+ * silent/recver.cu:124-163 and dev_layer.cu:62-143 like cuot_ggm_recver. */
+size_t cuot_ggm_recver_bs_work_bytes(const cuot_ggm_params *p);
+int cuot_ggm_recver_bs(cuot_ctx *ctx, const cuot_ggm_params *p, const uint8_t *choice_d, const cuot_block *csum_d,
+                       const cuot_block *fin_d, cuot_block *leaves_d, void *work_d, size_t work_bytes, void *stream);
 
 /* ---- Ferret 10-local primal LPN -------------------------------------------
  * out[i] = noise[i] ^ XOR_{j<10} kk[idx(i,j)], indices from AES under `seed`

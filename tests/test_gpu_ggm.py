@@ -155,10 +155,11 @@ def test_ggm_sender_receiver(ctx, h, t, mode):
         assert np.array_equal(w, ow)
 
 
-# the sender on the bitsliced cipher (breadth-first, one launch per level)
-# equals the LDS T-table sender: leaves, truncation, Ferret LSB rule and the
-# level sums of every tree, for levels with fewer and more than 32 nodes per
-# tree and for trees sharing and spanning waves
+# the sender and receiver on the bitsliced cipher (breadth-first, one launch
+# per level) equal the LDS T-table ones: leaves, truncation, Ferret LSB rule,
+# the level sums of every tree and the receiver's punctured-path rules, for
+# levels with fewer and more than 32 nodes per tree and for trees sharing and
+# spanning waves
 @pytest.mark.parametrize("h,t", [(1, 3), (3, 2), (5, 7), (6, 64), (9, 5), (11, 3), (13, 2), (15, 2)])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_ggm_sender_bitsliced_matches_ttable(ctx, h, t, mode):
@@ -180,3 +181,21 @@ def test_ggm_sender_bitsliced_matches_ttable(ctx, h, t, mode):
     if mode == 0 and t <= 3:
         lv, l, rr = po.ggm_sender_tree(k0, k1, roots[0], h)
         assert np.array_equal(to_np(b_leaves)[: 1 << h], lv)
+    # receiver from its own view, both ciphers
+    sums = to_np(a_sums).reshape(t, h, 2, 2)
+    ls, rs = sums[:, :, 0].reshape(-1, 2), sums[:, :, 1].reshape(-1, 2)
+    alphas = r.integers(0, 1 << h, size=t)
+    c = _alpha_choice(alphas, h)
+    csum = np.where(c.reshape(-1, 1) == 1, rs, ls)
+    delta = rand_blocks(r, 1)[0] | np.array([1, 0], dtype=np.uint64)
+    if mode == 1:
+        fin = rand_blocks(r, t)  # the secret sums: any blocks exercise the final rule
+    else:
+        last_l, last_r = ls.reshape(t, h, 2)[:, h - 1], rs.reshape(t, h, 2)[:, h - 1]
+        fin = np.where(c[:, h - 1:h] == 1, last_l, last_r) ^ delta
+    cd = torch.from_numpy(c.reshape(-1).copy()).cuda()
+    wa, wb = dev_blocks(n_out), dev_blocks(n_out)
+    ctx.ggm_recver(p, cd, to_dev(csum), to_dev(fin), wa)
+    rwork = torch.empty(ctx.ggm_recver_bs_work_bytes(p), dtype=torch.uint8, device="cuda")
+    ctx.ggm_recver_bs(p, cd, to_dev(csum), to_dev(fin), wb, rwork)
+    assert np.array_equal(to_np(wb)[:n_out], to_np(wa)[:n_out])

@@ -328,7 +328,8 @@ CB_HD void cbp_hi_terms(uint32_t& corr, const uint32_t* w, int base, int hb, int
   }
 }
 
-// hi_fix of level K (>= 10) on the shared string, 32-bit indices
+// hi_fix of level K (>= 10) on the shared string, 32-bit indices, one word per
+// task (the reference form for cbp_hi_fix4)
 template <int K, bool PAD>
 CB_HD void cbp_hi_fix(uint32_t* w, int base, int nw, int tid, int nth) {
   constexpr int JM = cbt_jmax<K>();
@@ -342,6 +343,73 @@ CB_HD void cbp_hi_fix(uint32_t* w, int base, int nw, int tid, int nth) {
     uint32_t corr = 0;
     cbp_hi_terms<K, PAD>(corr, w, base, hb, hw, wi);
     w[base + cbp_at<PAD>(hb + wi)] ^= corr;
+  }
+}
+
+// Correction quad for hi words 4q..4q+3 of the chunk whose hi half starts at
+// word hb: terms J >= 7 read the aligned quad ew words below the half's end
+// (all four words or none, since ew is a multiple of 4); terms J <= 6 touch
+// only words 0 and 1 and read the half's last quad T.
+template <int K, bool PAD, int J = 0>
+CB_HD void cbp_hi_terms4(cbt_w4& c, const uint32_t* w, int base, int hb, int hw, int q, const cbt_w4& T) {
+  if constexpr (J < K) {
+    if constexpr ((J & ~K) == 0) {
+      if constexpr (J >= 7) {
+        constexpr int ew = 1 << (J - 5);
+        if (4 * q < ew) {
+          const cbt_w4 Q = *(const cbt_w4*)(w + base + cbp_at<PAD>(hb + 4 * q + hw - ew));
+          cbt_x4(c, Q.x, Q.y, Q.z, Q.w);
+        }
+      } else if constexpr (J == 6) {
+        if (q == 0) {
+          c.x ^= T.z;
+          c.y ^= T.w;
+        }
+      } else if constexpr (J == 5) {
+        if (q == 0) c.x ^= T.w;
+      } else {
+        if (q == 0) c.x ^= T.w >> (32 - (1 << J));
+      }
+    }
+    cbp_hi_terms4<K, PAD, J + 1>(c, w, base, hb, hw, q, T);
+  }
+}
+
+// hi_fix of level K (>= 10) in 16-byte quads, two tasks per lane in flight:
+// every read comes from hi words >= hw / 2, which the fix never writes (the
+// fixed words are those below e_max <= hw / 2), so the loads of both tasks
+// precede both stores.  Same result as cbp_hi_fix.
+template <int K, bool PAD>
+CB_HD void cbp_hi_fix4(uint32_t* w, int base, int nw, int tid, int nth) {
+  constexpr int JM = cbt_jmax<K>();
+  static_assert(K >= 10 && JM >= 0, "levels >= 10 have lower terms");
+  constexpr int lhw = K - 5;
+  constexpr int hw = 1 << lhw;
+  constexpr int lnq = JM >= 7 ? JM - 7 : 0;  // log2 quads per chunk
+  const int total = (nw >> (lhw + 1)) << lnq;
+  for (int t0 = tid; t0 < total; t0 += 2 * nth) {
+    int hb[2], q[2];
+    bool live[2];
+    cbt_w4 c[2], H[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int t = t0 + u * nth;
+      live[u] = t < total;
+      const int tt = live[u] ? t : t0;  // an in-range task either way
+      const int g = tt >> lnq;
+      q[u] = tt & ((1 << lnq) - 1);
+      hb[u] = (g << (lhw + 1)) + hw;
+      const cbt_w4 T = *(const cbt_w4*)(w + base + cbp_at<PAD>(hb[u] + hw - 4));
+      c[u] = cbt_w4{0u, 0u, 0u, 0u};
+      cbp_hi_terms4<K, PAD>(c[u], w, base, hb[u], hw, q[u], T);
+      H[u] = *(const cbt_w4*)(w + base + cbp_at<PAD>(hb[u] + 4 * q[u]));
+    }
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      if (!live[u]) continue;
+      cbt_x4(H[u], c[u].x, c[u].y, c[u].z, c[u].w);
+      *(cbt_w4*)(w + base + cbp_at<PAD>(hb[u] + 4 * q[u])) = H[u];
+    }
   }
 }
 
@@ -511,13 +579,13 @@ CB_HD void cbp_level(uint32_t* w, int base, int64_t nw, int k, bool inverse, int
 #define CBP_CASE(K)                                                  \
   case K:                                                            \
     if (!inverse) {                                                  \
-      cbp_hi_fix<K, PAD>(w, base, (int)nw, (int)tid, (int)nth);      \
+      cbp_hi_fix4<K, PAD>(w, base, (int)nw, (int)tid, (int)nth);     \
       sync();                                                        \
       cbq_lo_update<K, PAD>(w, base, (int)nw, (int)tid, (int)nth);   \
     } else {                                                         \
       cbq_lo_update<K, PAD>(w, base, (int)nw, (int)tid, (int)nth);   \
       sync();                                                        \
-      cbp_hi_fix<K, PAD>(w, base, (int)nw, (int)tid, (int)nth);      \
+      cbp_hi_fix4<K, PAD>(w, base, (int)nw, (int)tid, (int)nth);     \
     }                                                                \
     break;
   switch (k) {

@@ -105,25 +105,36 @@ constexpr uint32_t cbt_mask_below(int j, int k) {
   return m;
 }
 
+// One level in place, one shift and one masked xor per term: with
+// half = 2^K and e = 2^J, lo bit j (e <= j < half) takes hi bit j - e, i.e.
+// word bit j + half - e, and hi bit i (i < e) takes hi bit i + half - e, word
+// bit i + 2 half - e: both are x >> (half - e) under a mask.  The lo terms
+// read hi bits only; the hi terms write hi bits below half + e <= 3 half / 2
+// and read hi bits at or above 2 half - e >= 3 half / 2; so every term reads
+// bits no term of its group writes, and the terms of a group commute.
 template <int K, int J = 0>
-CB_HD uint32_t cbt_word_corr(uint32_t h) {
+CB_HD uint32_t cbt_word_lo_ip(uint32_t x) {
   if constexpr (J >= K) {
-    return 0u;
+    return x;
   } else {
-    uint32_t c = cbt_word_corr<K, J + 1>(h);
-    if constexpr ((J & ~K) == 0) c ^= (h >> ((1 << K) - (1 << J))) & cbt_mask_below(J, K);
-    return c;
+    if constexpr ((J & ~K) == 0) {
+      constexpr uint32_t m = cbt_mask_lo(K) & ~cbt_mask_below(J, K);
+      x ^= (x >> ((1 << K) - (1 << J))) & m;
+    }
+    return cbt_word_lo_ip<K, J + 1>(x);
   }
 }
 
 template <int K, int J = 0>
-CB_HD uint32_t cbt_word_lo(uint32_t h) {
+CB_HD uint32_t cbt_word_hi_ip(uint32_t x) {
   if constexpr (J >= K) {
-    return 0u;
+    return x;
   } else {
-    uint32_t c = cbt_word_lo<K, J + 1>(h);
-    if constexpr ((J & ~K) == 0) c ^= (h << (1 << J)) & cbt_mask_lo(K);
-    return c;
+    if constexpr ((J & ~K) == 0) {
+      constexpr uint32_t m = cbt_mask_below(J, K) << (1 << K);
+      x ^= (x >> ((1 << K) - (1 << J))) & m;
+    }
+    return cbt_word_hi_ip<K, J + 1>(x);
   }
 }
 
@@ -131,18 +142,12 @@ template <int K, bool INV>
 CB_HD uint32_t cbt_word_level(uint32_t x) {
   if constexpr (K == 0) {
     return x;  // s_0(x) = x^1: no lower terms
+  } else if constexpr (!INV) {
+    // division: fix the hi half first, then the lo half from the fixed hi
+    return cbt_word_lo_ip<K>(cbt_word_hi_ip<K>(x));
   } else {
-    constexpr uint32_t mlo = cbt_mask_lo(K);
-    uint32_t h = (x >> (1 << K)) & mlo;
-    uint32_t l = x & mlo;
-    if constexpr (!INV) {
-      h ^= cbt_word_corr<K>(h);
-      l ^= cbt_word_lo<K>(h);
-    } else {
-      l ^= cbt_word_lo<K>(h);
-      h ^= cbt_word_corr<K>(h);
-    }
-    return l | (h << (1 << K));
+    // multiplication: the lo half from the original hi, then the hi fix
+    return cbt_word_hi_ip<K>(cbt_word_lo_ip<K>(x));
   }
 }
 

@@ -158,32 +158,33 @@ struct GgmBsFix {
   blk z0, z1;  // children of a zero parent
 };
 
+// one tree per lane, grid-stride (the grid is capped at small_grid's size)
 __global__ void ggm_bs_fix(GgmBsFix a) {
-  const int64_t tree = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (tree >= a.t) return;
-  const int64_t p = a.punct[tree];
-  const blk L = bxor(a.sums[(tree * a.h + a.d) * 2], a.z0);
-  const blk R = bxor(a.sums[(tree * a.h + a.d) * 2 + 1], a.z1);
-  const int c = a.choice[tree * a.h + a.d] ? 1 : 0;
-  const int64_t sib = 2 * p + c, np = 2 * p + 1 - c;
-  const blk sibv = bxor(a.csum[tree * a.h + a.d], c ? R : L);
-  const int64_t base = tree << (a.d + 1);
-  if (!a.last) {
-    a.out[base + sib] = sibv;
-    a.out[base + np] = bzero();
-    a.punct[tree] = np;
-    return;
+  for (int64_t tree = (int64_t)blockIdx.x * 256 + threadIdx.x; tree < a.t; tree += (int64_t)gridDim.x * 256) {
+    const int64_t p = a.punct[tree];
+    const blk L = bxor(a.sums[(tree * a.h + a.d) * 2], a.z0);
+    const blk R = bxor(a.sums[(tree * a.h + a.d) * 2 + 1], a.z1);
+    const int c = a.choice[tree * a.h + a.d] ? 1 : 0;
+    const int64_t sib = 2 * p + c, np = 2 * p + 1 - c;
+    const blk sibv = bxor(a.csum[tree * a.h + a.d], c ? R : L);
+    const int64_t base = tree << (a.d + 1);
+    if (!a.last) {
+      a.out[base + sib] = sibv;
+      a.out[base + np] = bzero();
+      a.punct[tree] = np;
+      continue;
+    }
+    blk v;
+    if (!a.ferret) {
+      v = bxor(a.fin[tree], c ? L : R);
+    } else {
+      blk o = bxor(bxor(L, R), sibv);
+      o.x &= ~1u;
+      v = bxor(a.fin[tree], o);
+    }
+    if (base + sib < a.n_out) a.out[base + sib] = a.ferret ? make_uint4(sibv.x & ~1u, sibv.y, sibv.z, sibv.w) : sibv;
+    if (base + np < a.n_out) a.out[base + np] = v;
   }
-  blk v;
-  if (!a.ferret) {
-    v = bxor(a.fin[tree], c ? L : R);
-  } else {
-    blk o = bxor(bxor(L, R), sibv);
-    o.x &= ~1u;
-    v = bxor(a.fin[tree], o);
-  }
-  if (base + sib < a.n_out) a.out[base + sib] = a.ferret ? make_uint4(sibv.x & ~1u, sibv.y, sibv.z, sibv.w) : sibv;
-  if (base + np < a.n_out) a.out[base + np] = v;
 }
 
 __global__ void ggm_bs_zero64(int64_t* p, int64_t n) {

@@ -93,3 +93,21 @@ def test_knobs_refuse_bad_values_and_set_the_crossover(lib):
     finally:
         tune.set_knobs()
     assert int(lib.cuot_qc_mfma_crossover()) == base
+
+
+def test_ggm_bitsliced_work_sizes(lib):
+    """The bitsliced GGM work queries: two levels of t * 2^(h-1) blocks for the
+    sender; the receiver adds t zero roots, t * h * 2 level sums and t
+    punctured indices; malformed trees give 0."""
+    import cuot_amd
+
+    B = cuot_amd.to_block
+    for t, h in [(1, 1), (3, 5), (1024, 15), (7, 16)]:
+        p = cuot_amd.GgmParams(t=t, h=h, mode=0, n_out=t << h, k0=B((1, 0)), k1=B((2, 0)))
+        s = lib.cuot_ggm_sender_bs_work_bytes(C.byref(p))
+        assert s == 2 * (t << (h - 1)) * 16, (t, h)
+        assert lib.cuot_ggm_recver_bs_work_bytes(C.byref(p)) == s + 16 * t + 32 * t * h + 8 * t, (t, h)
+    for t, h, mode in [(0, 5, 0), (3, 0, 0), (3, 17, 0), (3, 5, 7)]:
+        p = cuot_amd.GgmParams(t=t, h=h, mode=mode, n_out=0, k0=B((1, 0)), k1=B((2, 0)))
+        assert lib.cuot_ggm_sender_bs_work_bytes(C.byref(p)) == 0, (t, h, mode)
+        assert lib.cuot_ggm_recver_bs_work_bytes(C.byref(p)) == 0, (t, h, mode)

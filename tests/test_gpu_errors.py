@@ -40,6 +40,15 @@ def test_malformed_instances_refused(ctx):
     # GGM depth beyond 16
     gp = cuot_amd.GgmParams(t=1, h=17, mode=0, n_out=1 << 17, k0=B(SOT_K0), k1=B(SOT_K1))
     assert lib.cuot_ggm_sender(ctx.handle, C.byref(gp), p, p, p, None) == ERR_ARG
+    # bitsliced GGM forms with a work buffer one byte short of their query
+    gp = cuot_amd.GgmParams(t=3, h=5, mode=0, n_out=3 << 5, k0=B(SOT_K0), k1=B(SOT_K1))
+    need_s = lib.cuot_ggm_sender_bs_work_bytes(C.byref(gp))
+    need_r = lib.cuot_ggm_recver_bs_work_bytes(C.byref(gp))
+    assert 0 < need_s < need_r <= 1 << 20
+    assert lib.cuot_ggm_sender_bs(ctx.handle, C.byref(gp), p, p, p, p, need_s - 1, None) == ERR_ARG
+    assert b"work buffer" in lib.cuot_last_error()
+    assert lib.cuot_ggm_recver_bs(ctx.handle, C.byref(gp), p, p, p, p, p, need_r - 1, None) == ERR_ARG
+    assert b"work buffer" in lib.cuot_last_error()
     torch.cuda.synchronize()
 
 

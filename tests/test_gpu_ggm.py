@@ -86,12 +86,9 @@ def _params(t, h, mode, n_out, k0, k1):
 
 
 def _alpha_choice(alphas, h):
-    t = len(alphas)
-    c = np.zeros((t, h), dtype=np.uint8)
-    for tr, a in enumerate(alphas):
-        for d in range(h):
-            c[tr, d] = 1 - ((int(a) >> (h - 1 - d)) & 1)
-    return c
+    a = np.asarray(alphas, dtype=np.int64).reshape(-1, 1)
+    shifts = (h - 1 - np.arange(h, dtype=np.int64)).reshape(1, -1)
+    return (1 - ((a >> shifts) & 1)).astype(np.uint8)
 
 
 CASES = [(1, 3), (3, 2), (8, 3), (9, 2), (11, 3), (13, 2), (14, 2), (15, 2), (16, 1)]
@@ -159,8 +156,9 @@ def test_ggm_sender_receiver(ctx, h, t, mode):
 # per level) equal the LDS T-table ones: leaves, truncation, Ferret LSB rule,
 # the level sums of every tree and the receiver's punctured-path rules, for
 # levels with fewer and more than 32 nodes per tree and for trees sharing and
-# spanning waves
-@pytest.mark.parametrize("h,t", [(1, 3), (3, 2), (5, 7), (6, 64), (9, 5), (11, 3), (13, 2), (15, 2)])
+# spanning waves, and more trees than one grid of the per-tree kernels covers
+@pytest.mark.parametrize("h,t", [(1, 3), (3, 2), (5, 7), (6, 64), (9, 5), (11, 3), (13, 2), (15, 2),
+                                 (1, (1 << 20) + 3)])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_ggm_sender_bitsliced_matches_ttable(ctx, h, t, mode):
     import torch

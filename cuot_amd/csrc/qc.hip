@@ -292,19 +292,25 @@ __device__ __forceinline__ void rows_store(uint32_t* g, const uint32_t* sw, int6
     auto quad = [&](int64_t i) { return *(const uint4*)(sw + cbp_at<PAD>(i & wm)); };
     for (int f = lt; f < n4; f += tps) {
       const int64_t w = 4 * (int64_t)f;
-      uint4 acc = quad(w);
+      // the terms below one word (e < 32) all shift the quad at w with the
+      // word before it: both are read once
+      const uint4 q0 = quad(w);
+      const uint32_t pv0 = word(w - 1);
+      uint4 acc = q0;
       for (int j = 0; j < mp; j++) {
         if (j & ~mp) continue;
         const int64_t eo = ((int64_t)1 << j) >> 5;
         const int eb = j < 5 ? 1 << j : 0;
         uint4 c4;
-        if ((eo & 3) == 0) {
+        if (eo == 0) {
+          c4 = q0;
+        } else if ((eo & 3) == 0) {
           c4 = quad(w - eo);
         } else {
           c4 = make_uint4(word(w - eo), word(w - eo + 1), word(w - eo + 2), word(w - eo + 3));
         }
         if (eb) {
-          const uint32_t pv = word(w - eo - 1);
+          const uint32_t pv = pv0;  // eb != 0 only for j < 5, where eo = 0
           acc.x ^= (c4.x << eb) | (pv >> (32 - eb));
           acc.y ^= (c4.y << eb) | (c4.x >> (32 - eb));
           acc.z ^= (c4.z << eb) | (c4.y >> (32 - eb));

@@ -446,7 +446,39 @@ struct FftPassArgs {
   // next pass reads it (S0)
   const uint32_t* S0;
   uint32_t* D0;
+  // Plane layout of what the pass reads (gin) and writes (gout): 0 = plane
+  // major, element idx of plane p at block p*M + idx; 1 = planes grouped by
+  // four, at block ((p/4)*M + idx)*4 + p%4, so that the C consecutive
+  // columns of a tile are C*64 contiguous bytes of four planes.  Only the
+  // passes exchange the grouped form: the first pass reads the plane-major
+  // chunk planes, and the last pass writes plane-major into Xo.
+  int gin, gout;
+  uint32_t* Xo;  // destination of the planes not sent to D0 (X when NULL)
 };
+
+// 16-byte block of (plane gp, element idx) in plane-major (g = 0) or
+// grouped-by-four (g = 1) layout
+CUOT_DEV int64_t fft_xaddr(int g, int gp, int64_t idx, int64_t M) {
+  return g ? ((((int64_t)(gp >> 2)) * M + idx) << 2) | (int64_t)(gp & 3) : (int64_t)gp * M + idx;
+}
+
+// staged word f of a tile -> (plane p, tile row t, column c); the grouped
+// order keeps the four planes of a group and the columns innermost, so that
+// consecutive lanes touch consecutive blocks of the grouped layout
+CUOT_DEV void fft_split(int f, int g, int lgC, int R, int& c, int& t, int& p) {
+  if (g) {
+    const int q = f & 3;
+    c = (f >> 2) & ((1 << lgC) - 1);
+    const int r = f >> (2 + lgC);
+    t = r & ((1 << R) - 1);
+    p = ((r >> R) << 2) | q;
+  } else {
+    c = f & ((1 << lgC) - 1);
+    const int pt = f >> lgC;
+    t = pt & ((1 << R) - 1);
+    p = pt >> R;
+  }
+}
 
 // LDS words per tile element: NP planes x 4 words + 4 pad words (with the pad
 // the element stride is 4 mod 64 banks, so 16 consecutive elements' words of
@@ -639,7 +671,8 @@ __global__ __launch_bounds__(fthreads<NP>()) __attribute__((amdgpu_waves_per_eu(
   const int64_t ioff = (int64_t)blockIdx.y * a.XS_words;
   a.X += ioff;
   const uint4* S0 = (const uint4*)(a.S0 ? a.S0 + ioff : a.X);
-  uint4* D0 = (uint4*)(a.D0 ? a.D0 + ioff : a.X);
+  uint4* Y4 = (uint4*)(a.Xo ? a.Xo + ioff : a.X);
+  uint4* D0 = a.D0 ? (uint4*)(a.D0 + ioff) : Y4;
   if (a.ahats) a.ahat = a.ahats[blockIdx.y];
   const int h = NP == 16 ? (int)blockIdx.z : 0;  // half of a half-mode pass
   const int pbase = 16 * h;
@@ -672,15 +705,14 @@ __global__ __launch_bounds__(fthreads<NP>()) __attribute__((amdgpu_waves_per_eu(
       const int f = tid + u * NT;
       v[u] = z4();
       if (f < n4) {
-        const int c = f & (a.C - 1);
-        const int pt = f >> a.lgC;
-        const int t = pt & (T - 1), p = pt >> a.R;
+        int c, t, p;
+        fft_split(f, a.gin, a.lgC, a.R, c, t, p);
         const int64_t idx = (lhib << (a.lo + a.R)) | ((int64_t)t << a.lo) | (lcb * a.C + c);
         const int gp = pbase + p;  // global plane
         if (!a.first)
-          v[u] = (gp < 16 ? S0 : X4)[(int64_t)gp * a.M + idx];
+          v[u] = (gp < 16 ? S0 : X4)[fft_xaddr(a.gin, gp, idx, a.M)];
         else if (idx < half && p < 16)
-          v[u] = X4[(int64_t)p * a.M + idx];
+          v[u] = X4[(int64_t)p * a.M + idx];  // the chunk planes (plane major)
       }
     }
   };
@@ -699,9 +731,8 @@ __global__ __launch_bounds__(fthreads<NP>()) __attribute__((amdgpu_waves_per_eu(
     for (int u = 0; u < LD; u++) {
       const int f = (int)threadIdx.x + u * (int)blockDim.x;
       if (f < n4) {
-        const int c = f & (a.C - 1);
-        const int pt = f >> a.lgC;
-        const int t = pt & (T - 1), p = pt >> a.R;
+        int c, t, p;
+        fft_split(f, a.gin, a.lgC, a.R, c, t, p);
         *(uint4*)&lds[fslot(t * a.C + c) * ST + p * 4] = v[u];
       }
     }
@@ -735,24 +766,21 @@ __global__ __launch_bounds__(fthreads<NP>()) __attribute__((amdgpu_waves_per_eu(
       const int f = tid + u * NT;
       w[u] = z4();
       if (f < n4) {
-        const int c = f & (a.C - 1);
-        const int pt = f >> a.lgC;
-        const int t = pt & (T - 1);
-        w[u] = *(const uint4*)&lds[fslot(t * a.C + c) * ST + (pt >> a.R) * 4];
+        int c, t, p;
+        fft_split(f, a.gout, a.lgC, a.R, c, t, p);
+        w[u] = *(const uint4*)&lds[fslot(t * a.C + c) * ST + p * 4];
       }
     }
     if (tile + G < ntiles) load_tile(tile + G);
-    uint4* Y4 = (uint4*)a.X;
 #pragma unroll
     for (int u = 0; u < LD; u++) {
       const int f = tid + u * NT;
       if (f < n4) {
-        const int c = f & (a.C - 1);
-        const int pt = f >> a.lgC;
-        const int t = pt & (T - 1), p = pt >> a.R;
+        int c, t, p;
+        fft_split(f, a.gout, a.lgC, a.R, c, t, p);
         const int64_t idx = (hib << (a.lo + a.R)) | ((int64_t)t << a.lo) | (cb * a.C + c);
         const int gp = pbase + p;
-        (gp < 16 ? D0 : Y4)[(int64_t)gp * a.M + idx] = w[u];
+        (gp < 16 ? D0 : Y4)[fft_xaddr(a.gout, gp, idx, a.M)] = w[u];
       }
     }
     __syncthreads();
@@ -1288,6 +1316,9 @@ FftPassArgs pass_args(const Bufs& b, int64_t M, int lo, int R, int C, int mode, 
   a.XS_words = b.XS * 4;
   a.S0 = nullptr;
   a.D0 = nullptr;
+  a.gin = 0;
+  a.gout = 0;
+  a.Xo = nullptr;
   return a;
 }
 
@@ -1358,8 +1389,21 @@ __global__ __launch_bounds__(256) void qc_unpack_x(uint32_t* __restrict__ X, int
 // forward FFT layers m-1..0 (top layer from the packed lower half); with A^,
 // the lowest pass also multiplies by A^ and runs the inverse layers of its
 // range.  Returns the highest layer of the lowest pass in *low_khi.
+// The passes exchange X in the grouped layout when there are upper passes and
+// the first is in half mode: that pass reads the plane-major chunk planes
+// 0..15 while writing half 0 to the row buffer and half 1 to planes 16..31,
+// so no write lands on a block another tile still reads.  (A first pass in
+// full mode writes over its own input in place and keeps the plane-major
+// layout throughout.)  The last pass then writes plane-major to the idle row
+// buffer, and the caller swaps the two buffers' roles.
+bool fft_grouped(int m) {
+  const int R0 = m < FFT_LOGTC ? m : FFT_LOGTC;
+  if (m - 1 < R0) return false;  // one fused pass, in place
+  return upper_shape(m, m - 1, R0, true).half;
+}
+
 hipError_t fft_forward(const Bufs& b, int m, const uint32_t* ahat1, const uint32_t* const* ahats, bool fused,
-                       int* low_khi, hipStream_t st) {
+                       bool grouped, int* low_khi, hipStream_t st) {
   const int64_t M = (int64_t)1 << m;
   const int R0 = m < FFT_LOGTC ? m : FFT_LOGTC;
   int khi = m - 1;
@@ -1368,6 +1412,8 @@ hipError_t fft_forward(const Bufs& b, int m, const uint32_t* ahat1, const uint32
   while (khi >= R0) {
     const PassShape ps = upper_shape(m, khi, R0, true);
     FftPassArgs a = pass_args(b, M, ps.lo, ps.R, ps.C, PASS_FWD, khi, first ? 1 : 0, 0, nullptr, nullptr);
+    a.gin = (grouped && !first) ? 1 : 0;
+    a.gout = grouped ? 1 : 0;
     if (h0_in_rows) a.S0 = b.Bw;
     h0_in_rows = first && ps.half;
     if (h0_in_rows) a.D0 = b.Bw;
@@ -1381,13 +1427,16 @@ hipError_t fft_forward(const Bufs& b, int m, const uint32_t* ahat1, const uint32
   // the fused pass completes the inverse transform when no upper layers exist
   FftPassArgs a = pass_args(b, M, 0, R0, 1, fused ? PASS_FUSED : PASS_FWD, R0 - 1, first ? 1 : 0,
                             (fused && R0 == m) ? 1 : 0, ahat1, ahats);
+  a.gin = a.gout = grouped ? 1 : 0;  // grouped implies upper passes before and after this one
   if (h0_in_rows) a.S0 = b.Bw;
   launch_pass<32>(a, b.B, st);
   *low_khi = R0 - 1;
   return hipGetLastError();
 }
 
-hipError_t fft_inverse_upper(const Bufs& b, int m, int from, hipStream_t st) {
+// grouped: the last pass writes the plane-major result to the row buffer
+// (idle since the pass after the first), where the unpack runs as well
+hipError_t fft_inverse_upper(const Bufs& b, int m, int from, bool grouped, hipStream_t st) {
   const int64_t M = (int64_t)1 << m;
   int klo = from;
   while (klo <= m - 1) {
@@ -1395,11 +1444,15 @@ hipError_t fft_inverse_upper(const Bufs& b, int m, int from, hipStream_t st) {
     const bool last = ps.lo + ps.R == m;
     FftPassArgs a = pass_args(b, M, ps.lo, ps.R, ps.C, PASS_INV, ps.lo + ps.R - 1, 0, (last && !ps.half) ? 1 : 0,
                               nullptr, nullptr);
+    a.gin = grouped ? 1 : 0;
+    a.gout = (grouped && !last) ? 1 : 0;
+    if (grouped && last) a.Xo = b.Bw;
+    uint32_t* dst = (grouped && last) ? b.Bw : b.X;
     if (ps.half) {
       launch_pass<16>(a, b.B, st);
       if (last) {
         cuot_rec_begin(st, "qc_unpack_x", 2.0 * 512.0 * (double)M * b.B);
-        hipLaunchKernelGGL(qc_unpack_x, dim3(capg(M * 4 / 256, 8192), (unsigned)b.B), dim3(256), 0, st, b.X, M,
+        hipLaunchKernelGGL(qc_unpack_x, dim3(capg(M * 4 / 256, 8192), (unsigned)b.B), dim3(256), 0, st, dst, M,
                            b.XS * 4);
         cuot_rec_end(st);
       }
@@ -1424,8 +1477,16 @@ hipError_t encode_batch(const QcIo* io, QcIo io1, const uint32_t* ahat1, const u
   cuot_rec_end(st);
   if ((e = conv_forward(b, M, 16, m - 1, st))) return e;
   int khi;
-  if ((e = fft_forward(b, m, ahat1, ahats, true, &khi, st))) return e;
-  if ((e = fft_inverse_upper(b, m, khi + 1, st))) return e;
+  const bool grouped = fft_grouped(m);
+  if ((e = fft_forward(b, m, ahat1, ahats, true, grouped, &khi, st))) return e;
+  if ((e = fft_inverse_upper(b, m, khi + 1, grouped, st))) return e;
+  if (grouped) {
+    // the transform's result sits in the row buffer: the two buffers swap
+    // roles for the inverse conversion and the output
+    uint32_t* t = b.X;
+    b.X = b.Bw;
+    b.Bw = t;
+  }
   // n = 16 * M/2 (n a power of two): fold in the row layout and write y from it
   const bool fold = !io && io1.n == 8 * M && m >= 11 && row_path(m - 1) && B == 1;
   if ((e = conv_inverse(b, M, 31, m - 1, fold, st))) return e;

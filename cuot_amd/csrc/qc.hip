@@ -1372,17 +1372,23 @@ PassShape upper_shape(int m, int kfrom, int kto, bool down) {
 // unpack of the chunk products after a last inverse pass in half mode: the 32
 // coordinates of every element word become its 31 coefficient planes (plane 31
 // zero).  Plane p of instance i is words [(i * 32 + p) * 4M, ... + 4M).
-__global__ __launch_bounds__(256) void qc_unpack_x(uint32_t* __restrict__ X, int64_t M, int64_t XS_words) {
-  uint32_t* Xi = X + (int64_t)blockIdx.y * XS_words;
+// G: the source holds the planes grouped by four (block ((p/4)*M + e)*4 + p%4),
+// read from src and written plane-major to dst (a different buffer); else
+// src == dst, plane-major, in place.
+template <bool G>
+__global__ __launch_bounds__(256) void qc_unpack_x(const uint32_t* src, uint32_t* dst, int64_t M, int64_t XS_words) {
+  const uint32_t* Si = src + (int64_t)blockIdx.y * XS_words;
+  uint32_t* Di = dst + (int64_t)blockIdx.y * XS_words;
   const int64_t pw = M * 4;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < pw; t += (int64_t)gridDim.x * blockDim.x) {
     uint32_t in[32], out[31];
 #pragma unroll
-    for (int p = 0; p < 32; p++) in[p] = Xi[p * pw + t];
+    for (int p = 0; p < 32; p++)
+      in[p] = G ? Si[((((int64_t)(p >> 2)) * M + (t >> 2)) * 4 + (p & 3)) * 4 + (t & 3)] : Si[p * pw + t];
     gf32_unpack_planes(in, out);
 #pragma unroll
-    for (int p = 0; p < 31; p++) Xi[p * pw + t] = out[p];
-    Xi[31 * pw + t] = 0u;
+    for (int p = 0; p < 31; p++) Di[p * pw + t] = out[p];
+    Di[31 * pw + t] = 0u;
   }
 }
 
@@ -1434,9 +1440,13 @@ hipError_t fft_forward(const Bufs& b, int m, const uint32_t* ahat1, const uint32
   return hipGetLastError();
 }
 
-// grouped: the last pass writes the plane-major result to the row buffer
-// (idle since the pass after the first), where the unpack runs as well
-hipError_t fft_inverse_upper(const Bufs& b, int m, int from, bool grouped, hipStream_t st) {
+// grouped: the last pass writes to the row buffer (idle since the pass after
+// the first).  In half mode it writes the grouped form, and the unpack reads
+// it from there and writes the plane-major coefficient planes back to X; in
+// full mode it unpacks in the tile and writes plane-major, and *swapped tells
+// the caller that the result is in the row buffer.
+hipError_t fft_inverse_upper(const Bufs& b, int m, int from, bool grouped, bool* swapped, hipStream_t st) {
+  *swapped = false;
   const int64_t M = (int64_t)1 << m;
   int klo = from;
   while (klo <= m - 1) {
@@ -1445,19 +1455,23 @@ hipError_t fft_inverse_upper(const Bufs& b, int m, int from, bool grouped, hipSt
     FftPassArgs a = pass_args(b, M, ps.lo, ps.R, ps.C, PASS_INV, ps.lo + ps.R - 1, 0, (last && !ps.half) ? 1 : 0,
                               nullptr, nullptr);
     a.gin = grouped ? 1 : 0;
-    a.gout = (grouped && !last) ? 1 : 0;
+    a.gout = (grouped && (!last || ps.half)) ? 1 : 0;
     if (grouped && last) a.Xo = b.Bw;
-    uint32_t* dst = (grouped && last) ? b.Bw : b.X;
     if (ps.half) {
       launch_pass<16>(a, b.B, st);
       if (last) {
         cuot_rec_begin(st, "qc_unpack_x", 2.0 * 512.0 * (double)M * b.B);
-        hipLaunchKernelGGL(qc_unpack_x, dim3(capg(M * 4 / 256, 8192), (unsigned)b.B), dim3(256), 0, st, dst, M,
-                           b.XS * 4);
+        if (grouped)
+          hipLaunchKernelGGL(qc_unpack_x<true>, dim3(capg(M * 4 / 256, 8192), (unsigned)b.B), dim3(256), 0, st,
+                             b.Bw, b.X, M, b.XS * 4);
+        else
+          hipLaunchKernelGGL(qc_unpack_x<false>, dim3(capg(M * 4 / 256, 8192), (unsigned)b.B), dim3(256), 0, st,
+                             b.X, b.X, M, b.XS * 4);
         cuot_rec_end(st);
       }
     } else {
       launch_pass<32>(a, b.B, st);
+      if (grouped && last) *swapped = true;
     }
     klo += ps.R;
   }
@@ -1479,8 +1493,9 @@ hipError_t encode_batch(const QcIo* io, QcIo io1, const uint32_t* ahat1, const u
   int khi;
   const bool grouped = fft_grouped(m);
   if ((e = fft_forward(b, m, ahat1, ahats, true, grouped, &khi, st))) return e;
-  if ((e = fft_inverse_upper(b, m, khi + 1, grouped, st))) return e;
-  if (grouped) {
+  bool swapped;
+  if ((e = fft_inverse_upper(b, m, khi + 1, grouped, &swapped, st))) return e;
+  if (swapped) {
     // the transform's result sits in the row buffer: the two buffers swap
     // roles for the inverse conversion and the output
     uint32_t* t = b.X;
